@@ -1,0 +1,134 @@
+/*
+ * jacquard_hip.h -- C ABI of the MI355X (gfx950) Boolean-processor backend.
+ *
+ * libjacquard_hip.so exports the reference's two launch symbols with the
+ * same signatures and semantics, so the existing host binding links against
+ * it unchanged, plus new entry points for status-returning launches, the
+ * batched lane-sliced form, per-partition timing samples and the learned
+ * cost model's MLP.
+ *
+ * Conventions shared by every entry point:
+ *   - all pointers are device pointers owned by the caller; the library
+ *     allocates no memory on the caller's behalf except where stated;
+ *   - launches are asynchronous on the given stream (the reference symbols
+ *     use the null stream); the caller synchronises;
+ *   - "usize" is 64-bit, as the reference's Rust usize.
+ */
+#ifndef JACQUARD_HIP_H
+#define JACQUARD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef uint64_t usize;
+typedef uint32_t u32;
+typedef int32_t i32;
+typedef uint8_t u8;
+typedef uint16_t u16;
+
+/* ---- Reference drop-in symbols ------------------------------------------
+ * Replaces csrc/kernel_v1.hip.cpp:34-66 (and the CUDA twin
+ * csrc/kernel_v1.cu:17-47).  Runs num_cycles cycles of the packed script
+ * over states_noninteractive[(num_cycles+1) * state_size] in place: snapshot
+ * c is the input of cycle c, snapshot c+1 (primary inputs pre-filled by the
+ * caller) receives its outputs.  No timing checks.  On a launch error prints
+ * a message and exits with EXIT_FAILURE, as the reference does. */
+void simulate_v1_noninteractive_simple_scan_hip(
+    usize num_blocks, usize num_major_stages, const usize *blocks_start,
+    const u32 *blocks_data, u32 *sram_data, u32 *sram_xmask, usize num_cycles,
+    usize state_size, u32 *states_noninteractive, i32 arrival_state_offset);
+
+/* Replaces csrc/kernel_v1.hip.cpp:69-100 (CUDA twin csrc/kernel_v1.cu:50-78).
+ * As above, plus setup/hold checks: timing_constraints = [clock_ps,
+ * per-state-word (setup<<16 | hold)...] or NULL, event_buffer = the 16-byte
+ * header + 1024 x 32-byte Event layout of csrc/event_buffer.h:35-50 or NULL.
+ * Events beyond 1024 set the overflow flag. */
+void simulate_v1_noninteractive_timed_hip(
+    usize num_blocks, usize num_major_stages, const usize *blocks_start,
+    const u32 *blocks_data, u32 *sram_data, u32 *sram_xmask, usize num_cycles,
+    usize state_size, u32 *states_noninteractive,
+    const u32 *timing_constraints, u8 *event_buffer, i32 arrival_state_offset);
+
+/* ---- Status-returning form (no counterpart in the reference) ------------
+ * Same computation as the timed symbol with an explicit stream and an event
+ * capacity of the caller's choosing (event_buffer must hold
+ * 16 + 32*event_capacity bytes).  Returns 0 or a negative error code; the
+ * message is available from jq_last_error().  For hosts that must not exit
+ * on error (tests, services). */
+int jq_simulate_scan(usize num_blocks, usize num_major_stages,
+                     const usize *blocks_start, const u32 *blocks_data,
+                     u32 *sram_data, u32 *sram_xmask, usize num_cycles,
+                     usize state_size, u32 *states,
+                     const u32 *timing_constraints, u8 *event_buffer,
+                     u32 event_capacity, i32 arrival_state_offset,
+                     void *stream);
+
+/* Per-partition timing samples (no counterpart in the reference; feeds the
+ * learned cost model).  Runs like jq_simulate_scan and additionally writes,
+ * for every partition execution of the first sample_cycles cycles, its
+ * duration in GPU clock ticks into samples[cycle * num_parts + part], where
+ * part is the partition's order of appearance in blocks_data (stage-major,
+ * then block, then position in block).  samples is u32[sample_cycles *
+ * num_parts]. */
+int jq_simulate_scan_profiled(usize num_blocks, usize num_major_stages,
+                              const usize *blocks_start, const u32 *blocks_data,
+                              u32 *sram_data, u32 *sram_xmask,
+                              usize num_cycles, usize state_size, u32 *states,
+                              const u32 *part_index, u32 num_parts,
+                              u32 *samples, u32 sample_cycles, void *stream);
+
+/* ---- Batched lane-sliced form (no counterpart in the reference) ---------
+ * Simulates 32*num_groups independent stimulus lanes of one script in a
+ * single persistent launch.  State words are stored bit-transposed: for
+ * every state bit, one u32 whose bit l belongs to lane l of the group.
+ *   blocks_per_group : script blocks (== the script's num_blocks); the
+ *                      launch uses num_groups * blocks_per_group workgroups
+ *   group_cycles[g]  : cycles to run for group g
+ *   states           : num_groups * 2 * (reg_io_state_size*32) u32 ping-pong
+ *                      buffers (lane-words), snapshot 0 in buffer 0
+ *   sram_data        : num_groups * 32 * sram_storage_size u32 (per lane)
+ *   num_input_bits   : primary-input bits (the first bits of the state)
+ *   lane_seed/lane_thresh : per-lane stimulus RNG seed and toggle threshold
+ *                      (toggle when hash < thresh); 32*num_groups entries
+ *   clock_flag_bits  : positions set to 1 every cycle (posedge flags), n_flags
+ *   obs_bits/num_obs : state bits captured after every cycle into
+ *                      obs_out[g][cycle][k] (lane-words)
+ * Returns 0 or a negative error code. */
+int jq_simulate_batched(usize num_groups, usize blocks_per_group,
+                        usize num_major_stages, const usize *blocks_start,
+                        const u32 *blocks_data, u32 reg_io_state_size,
+                        u32 sram_storage_size, u32 *sram_data,
+                        const u32 *group_cycles, u32 max_cycles, u32 *states,
+                        u32 num_input_bits, const u32 *lane_seed,
+                        const u32 *lane_thresh, const u32 *clock_flag_bits,
+                        u32 n_flags, const u32 *obs_bits, u32 num_obs,
+                        u32 *obs_out, void *stream);
+
+/* Occupancy query used by the autotuner: the largest number of resident
+ * workgroups (all co-resident, as the persistent launches require) for the
+ * scan kernel (kind 0) or the batched kernel (kind 1) on the current device. */
+int jq_max_resident_blocks(int kind, int *out_blocks);
+
+/* ---- Learned cost model MLP (no counterpart in the reference) -----------
+ * Fully connected FP32 network in -> h1 -> h2 -> 1 with ReLU, row-major
+ * weights W1[h1][in], b1[h1], W2[h2][h1], b2[h2], W3[h2], b3[1] packed in
+ * that order in `params`.  Forward: y[n] for x[n][in].  Train step: one
+ * mean-squared-error gradient over the batch, written to `grads` (same
+ * packing), then params -= lr * grads.  All on `stream`. */
+int jq_mlp_forward(const float *params, int in, int h1, int h2,
+                   const float *x, int n, float *y, void *stream);
+int jq_mlp_train_step(float *params, float *grads, int in, int h1, int h2,
+                      const float *x, const float *target, int n, float lr,
+                      float *loss_out, void *stream);
+
+/* Last error message of this thread ("" if none). */
+const char *jq_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,71 @@
+// Shared device helpers for the gfx950 kernels: grid barrier, error state,
+// script-format constants.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "jacquard_hip.h"
+
+namespace jqd {
+
+constexpr int kThreads = 256;      // words per 8192-bit boomerang layer
+constexpr u32 kSramWords = 8192;   // one AIGPDK SRAM (src/aigpdk.rs:22)
+constexpr int kBarrierSlots = 256; // concurrent persistent launches
+
+// Barrier slots (count, generation) and a sticky error word.  Every launch
+// takes its own slot; generations only grow, counts return to zero after
+// every barrier, so no per-launch reset is needed.
+extern __device__ u32 g_barrier[kBarrierSlots * 2];
+extern __device__ u32 g_error;
+
+// Grid-wide barrier among `n` co-resident workgroups.  Thread 0 of each
+// workgroup arrives with a device-scope release and waits with a
+// device-scope acquire (these also write back / invalidate the per-XCD L2
+// state that other workgroups read).  The wait is bounded: after ~2^27
+// polls it raises g_error and returns so the grid always drains.
+__device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    u32 arrived = atomicAdd(&bar[0], 1u);
+    if (arrived == n - 1) {
+      atomicExch(&bar[0], 0u);
+      __threadfence();
+      atomicAdd(&bar[1], 1u);
+    } else {
+      u32 spins = 0;
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE,
+                               __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1u << 27)) {
+          atomicOr(&g_error, 1u);
+          break;
+        }
+      }
+    }
+    __threadfence();
+    gen += 1;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ u32 read_gen(const u32 *bar) {
+  return __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One AND-with-invert word of the boomerang tree
+// (csrc/kernel_v1_impl.cuh:209-211).
+__device__ __forceinline__ u32 gate(u32 a, u32 b, u32 xa, u32 xb, u32 ob) {
+  return (a ^ xa) & ((b ^ xb) | ob);
+}
+
+// Its X lane (kernel_v1_impl.cuh:214-219).
+__device__ __forceinline__ u32 gate_x(u32 a, u32 ax, u32 b, u32 bx, u32 xa,
+                                      u32 xb, u32 ob) {
+  u32 ae = a ^ xa, be = (b ^ xb) | ob, bex = bx & ~ob;
+  return (ax | bex) & (ae | ax) & (be | bex);
+}
+
+}  // namespace jqd

@@ -1,0 +1,553 @@
+// Plain drop-in interpreter for gfx950 (wave64, 4 waves per workgroup).
+//
+// One workgroup per script block, persistent over all cycles and major
+// stages, a grid barrier after every major stage (the contract of
+// csrc/kernel_v1_impl.cuh:565-626).  Inside a partition the work is mapped
+// to CDNA4 as follows:
+//   - script sections are streamed with 16-byte per-thread loads (each
+//     section is one fully coalesced 4 KiB request), the next boomerang
+//     stage's permutation words are in flight while the current tree runs;
+//   - the 8192-bit shuffle is 32 LDS bit-gathers per thread on all 4 waves;
+//   - the whole AND tree (hier 0..7 and the in-word tail) runs in wave 0
+//     with cross-lane moves, so a boomerang stage costs two workgroup
+//     barriers instead of one per tree level;
+//   - X masks and u16 arrivals ride in the same wave-0 pass when enabled.
+// The computed values are bit-identical to the reference CPU interpreter
+// (src/sim/cpu_reference.rs:17-674) and to the CUDA kernel's X, arrival and
+// setup/hold rules, with the writeout arrival reset at every partition.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "jq_common.hpp"
+
+namespace jqd {
+
+__device__ u32 g_barrier[kBarrierSlots * 2];
+__device__ u32 g_error;
+
+namespace {
+
+struct EvHdr {
+  u32 count, overflow, r0, r1;
+};
+struct Ev {
+  u32 type, msg, cycle, r;
+  u32 data[4];
+};
+
+struct ScanArgs {
+  u32 num_blocks, num_major_stages;
+  const usize *blocks_start;
+  const u32 *blocks_data;
+  u32 *sram_data, *sram_xmask;
+  u32 num_cycles;
+  usize state_size;
+  u32 *states;
+  const u32 *timing;       // [clock_ps, per-word constraints...] or null
+  const u32 *constraints;  // timing + 1, set on device
+  u32 clock_ps;            // timing[0], set on device
+  EvHdr *events;
+  u32 ev_cap;
+  i32 arrival_state_offset;
+  u32 *bar;
+  // profiling
+  const u32 *part_index;
+  u32 num_parts;
+  u32 *samples;
+  u32 sample_cycles;
+};
+
+__device__ __forceinline__ uint4 ld4(const u32 *p) {
+  return *reinterpret_cast<const uint4 *>(p);
+}
+
+// 32 LDS bit-gathers: word bit 2k (2k+1) <- src bit lo16 (hi16) of perm
+// word k (src/flatten.rs:1027-1034).
+__device__ __forceinline__ u32 gather_bits(const u32 *src, uint4 q0, uint4 q1,
+                                           uint4 q2, uint4 q3) {
+  u32 r = 0;
+  const u32 w[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                     q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    u32 i1 = w[k] & 0xffffu, i2 = w[k] >> 16;
+    r |= ((src[i1 >> 5] >> (i1 & 31)) & 1u) << (2 * k);
+    r |= ((src[i2 >> 5] >> (i2 & 31)) & 1u) << (2 * k + 1);
+  }
+  return r;
+}
+
+__device__ __forceinline__ u16 arr_step(u16 a, u16 b, u32 ob, u32 dly) {
+  u16 base = (ob == 0xFFFFFFFFu) ? a : (a > b ? a : b);
+  return (u16)(base + (u16)(dly & 0xffffu));
+}
+
+__device__ __forceinline__ void push_event(EvHdr *ev, u32 cap, u32 type,
+                                           u32 word, u32 cycle, int slack,
+                                           u32 arrival, u32 lim) {
+  u32 k = atomicAdd(&ev->count, 1u);
+  Ev *e = reinterpret_cast<Ev *>(ev + 1);
+  if (k < cap) {
+    e[k].type = type;
+    e[k].msg = word;
+    e[k].cycle = cycle;
+    e[k].data[0] = word;
+    e[k].data[1] = (u32)slack;
+    e[k].data[2] = arrival;
+    e[k].data[3] = lim;
+  } else {
+    atomicExch(&ev->overflow, 1u);
+  }
+}
+
+template <bool TIMING, bool PROFILE>
+__device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
+                          const u32 *__restrict__ in, u32 *__restrict__ out,
+                          u32 cycle, u32 part_base, u32 *st_v, u32 *st_x,
+                          u32 *hin_v, u32 *hin_x, u32 *wo_v, u32 *wo_x,
+                          u16 *st_arr, u16 *wo_arr) {
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const bool w0 = t < 64;
+  size_t pi = 0;
+  u32 part = part_base;
+  for (;;) {
+    const u32 num_stages = script[pi + 0];
+    if (num_stages == 0) break;
+    long long t_start = 0;
+    if (PROFILE) t_start = clock64();
+    const u32 is_last = script[pi + 1];
+    const u32 num_ios = script[pi + 2];
+    const u32 io_offset = script[pi + 3];
+    const u32 num_srams = script[pi + 4];
+    const u32 sram_offset = script[pi + 5];
+    const u32 rounds = script[pi + 6];
+    const u32 num_dups = script[pi + 7];
+    const bool is_x = script[pi + 8] != 0;
+    const u32 xoff = script[pi + 9];
+    u32 hook = script[pi + 128 + (t >> 1)];
+    hook = (t & 1) ? (hook >> 16) : (hook & 0xffffu);
+    pi += 256;
+
+    // ---- global read (pext-gather), kernel_v1_impl.cuh:87-151
+    u32 acc_v = 0, acc_x = 0;
+    for (u32 r = 0; r < rounds; r += 4) {
+      uint2 e[4];
+      u32 v[4], x[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        e[k] = (r + k < rounds)
+                   ? reinterpret_cast<const uint2 *>(script + pi + 512 * (r + k))[t]
+                   : make_uint2(0, 0);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        v[k] = 0;
+        x[k] = 0;
+        if (e[k].y) {
+          const u32 *base = (e[k].x >> 31) ? out : in;
+          u32 w = e[k].x & 0x7fffffffu;
+          v[k] = base[w];
+          if (is_x) x[k] = base[xoff + w];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        u32 m = e[k].y;
+        while (m) {
+          u32 low = m & (0u - m);
+          acc_v = (acc_v << 1) | ((v[k] & low) ? 1u : 0u);
+          acc_x = (acc_x << 1) | ((x[k] & low) ? 1u : 0u);
+          m ^= low;
+        }
+      }
+    }
+    pi += (size_t)512 * rounds;
+    st_v[t] = acc_v;
+    st_x[t] = acc_x;
+    if (TIMING) wo_arr[t] = 0;  // pinned rule: reset per partition
+    __syncthreads();
+
+    // ---- boomerang stages
+    uint4 q0 = ld4(script + pi + 4 * t), q1 = ld4(script + pi + 1024 + 4 * t);
+    uint4 q2 = ld4(script + pi + 2048 + 4 * t), q3 = ld4(script + pi + 3072 + 4 * t);
+    for (u32 bs = 0; bs < num_stages; ++bs) {
+      u32 hv = gather_bits(st_v, q0, q1, q2, q3);
+      u32 hx = is_x ? gather_bits(st_x, q0, q1, q2, q3) : 0u;
+      const u32 *fl = script + pi + 4096;
+      uint4 fA, fB, fC, fD;
+      if (w0) {
+        fA = ld4(fl + 4 * lane);
+        fB = ld4(fl + 4 * (64 + lane));
+        fC = ld4(fl + 4 * (128 + lane));
+        fD = ld4(fl + 4 * (192 + lane));
+      }
+      pi += 5120;
+      if (bs + 1 < num_stages) {
+        q0 = ld4(script + pi + 4 * t);
+        q1 = ld4(script + pi + 1024 + 4 * t);
+        q2 = ld4(script + pi + 2048 + 4 * t);
+        q3 = ld4(script + pi + 3072 + 4 * t);
+      }
+      hin_v[t] = hv;
+      hin_x[t] = hx;
+      __syncthreads();
+      if (w0) {
+        // hier0: words 128+L, 192+L; hier1: word 64+L
+        u32 s0 = hin_v[lane], s1 = hin_v[64 + lane];
+        u32 s2 = hin_v[128 + lane], s3 = hin_v[192 + lane];
+        u32 o2 = gate(s0, s2, fC.x, fC.y, fC.z);
+        u32 o3 = gate(s1, s3, fD.x, fD.y, fD.z);
+        u32 o1 = gate(o2, o3, fB.x, fB.y, fB.z);
+        u32 o2x = 0, o3x = 0, o1x = 0, curx = 0;
+        if (is_x) {
+          u32 x0 = hin_x[lane], x1 = hin_x[64 + lane];
+          u32 x2 = hin_x[128 + lane], x3 = hin_x[192 + lane];
+          o2x = gate_x(s0, x0, s2, x2, fC.x, fC.y, fC.z);
+          o3x = gate_x(s1, x1, s3, x3, fD.x, fD.y, fD.z);
+          o1x = gate_x(o2, o2x, o3, o3x, fB.x, fB.y, fB.z);
+        }
+        u16 a2 = 0, a3 = 0, a1 = 0, cura = 0;
+        if (TIMING) {
+          a2 = arr_step(0, 0, fC.z, fC.w);
+          a3 = arr_step(0, 0, fD.z, fD.w);
+          a1 = arr_step(a2, a3, fB.z, fB.w);
+        }
+        // hier2: lanes 32..63 -> word L from words L+32 (lane L-32) and
+        // L+64 (own hier1 word)
+        u32 cur = 0;
+        {
+          u32 a = __shfl(o1, (lane + 32) & 63);
+          u32 ax = is_x ? __shfl(o1x, (lane + 32) & 63) : 0u;
+          u32 aa = TIMING ? (u32)__shfl((int)a1, (lane + 32) & 63) : 0u;
+          if (lane >= 32) {
+            cur = gate(a, o1, fA.x, fA.y, fA.z);
+            if (is_x) curx = gate_x(a, ax, o1, o1x, fA.x, fA.y, fA.z);
+            if (TIMING) cura = arr_step((u16)aa, a1, fA.z, fA.w);
+          }
+        }
+        // hier3..7: lanes [hw,2hw) from lanes L+hw and L+2hw
+#pragma unroll
+        for (int hw = 16; hw >= 1; hw >>= 1) {
+          u32 a = __shfl(cur, (lane + hw) & 63);
+          u32 b = __shfl(cur, (lane + 2 * hw) & 63);
+          u32 ax = 0, bx = 0, aa = 0, ba = 0;
+          if (is_x) {
+            ax = __shfl(curx, (lane + hw) & 63);
+            bx = __shfl(curx, (lane + 2 * hw) & 63);
+          }
+          if (TIMING) {
+            aa = (u32)__shfl((int)cura, (lane + hw) & 63);
+            ba = (u32)__shfl((int)cura, (lane + 2 * hw) & 63);
+          }
+          if (lane >= hw && lane < 2 * hw) {
+            cur = gate(a, b, fA.x, fA.y, fA.z);
+            if (is_x) curx = gate_x(a, ax, b, bx, fA.x, fA.y, fA.z);
+            if (TIMING) cura = arr_step((u16)aa, (u16)ba, fA.z, fA.w);
+          }
+        }
+        // in-word tail hier8..12 on lane 0 (kernel_v1_impl.cuh:294-346)
+        u32 v1 = __shfl(cur, 1);
+        u32 v1x = is_x ? __shfl(curx, 1) : 0u;
+        if (lane == 0) {
+          u32 xa = fA.x, xb = fA.y, ob = fA.z;
+          u32 r8 = ((v1 << 16) ^ xa) & ((v1 ^ xb) | ob) & 0xffff0000u;
+          u32 r9 = ((r8 >> 8) ^ xa) & (((r8 >> 16) ^ xb) | ob) & 0xff00u;
+          u32 r10 = ((r9 >> 4) ^ xa) & (((r9 >> 8) ^ xb) | ob) & 0xf0u;
+          u32 r11 = ((r10 >> 2) ^ xa) & (((r10 >> 4) ^ xb) | ob) & 0xcu;
+          u32 r12 = ((r11 >> 1) ^ xa) & (((r11 >> 2) ^ xb) | ob) & 0x2u;
+          cur = r8 | r9 | r10 | r11 | r12;
+          if (is_x) {
+            u32 x8 = gate_x(v1 << 16, v1x << 16, v1, v1x, xa, xb, ob) & 0xffff0000u;
+            u32 x9 = gate_x(r8 >> 8, x8 >> 8, r8 >> 16, x8 >> 16, xa, xb, ob) & 0xff00u;
+            u32 x10 = gate_x(r9 >> 4, x9 >> 4, r9 >> 8, x9 >> 8, xa, xb, ob) & 0xf0u;
+            u32 x11 = gate_x(r10 >> 2, x10 >> 2, r10 >> 4, x10 >> 4, xa, xb, ob) & 0xcu;
+            u32 x12 = gate_x(r11 >> 1, x11 >> 1, r11 >> 2, x11 >> 2, xa, xb, ob) & 0x2u;
+            curx = x8 | x9 | x10 | x11 | x12;
+          }
+          cura = 0;
+        }
+        st_v[128 + lane] = o2;
+        st_v[192 + lane] = o3;
+        st_v[64 + lane] = o1;
+        st_v[lane] = cur;
+        if (is_x) {
+          st_x[128 + lane] = o2x;
+          st_x[192 + lane] = o3x;
+          st_x[64 + lane] = o1x;
+          st_x[lane] = curx;
+        }
+        if (TIMING) {
+          st_arr[128 + lane] = a2;
+          st_arr[192 + lane] = a3;
+          st_arr[64 + lane] = a1;
+          st_arr[lane] = cura;
+        }
+      }
+      __syncthreads();
+      if ((hook >> 8) == bs) {
+        wo_v[t] = st_v[hook & 255];
+        if (is_x) wo_x[t] = st_x[hook & 255];
+        if (TIMING) wo_arr[t] = st_arr[hook & 255];
+      }
+    }
+    __syncthreads();
+
+    // ---- SRAM + duplicate permutation, kernel_v1_impl.cuh:365-443
+    const u32 nperm = num_srams * 4 + num_dups;
+    u32 pv = 0, px = 0;
+    {
+      uint4 p0 = ld4(script + pi + 4 * t), p1 = ld4(script + pi + 1024 + 4 * t);
+      uint4 p2 = ld4(script + pi + 2048 + 4 * t), p3 = ld4(script + pi + 3072 + 4 * t);
+      uint4 f = ld4(script + pi + 4096 + 4 * t);
+      if ((u32)t < nperm) {
+        pv = (gather_bits(wo_v, p0, p1, p2, p3) & ~f.y) ^ f.x;
+        if (is_x) px = gather_bits(wo_x, p0, p1, p2, p3) & ~f.y;
+      }
+    }
+    pi += 5120;
+    const u32 en = __shfl(pv, (lane + 1) & 63);
+    const u32 wdata = __shfl(pv, (lane + 2) & 63);
+    const u32 wdata_x = __shfl(px, (lane + 2) & 63);
+    __syncthreads();  // every gather of wo_* is done before it is written
+    if ((u32)t < num_srams * 4) {
+      if ((t & 3) == 0) {
+        u32 s = (u32)t >> 2;
+        size_t st = (size_t)sram_offset + (size_t)s * kSramWords;
+        u32 ra = (pv & 0xffffu) & (kSramWords - 1);
+        u32 wa = (pv >> 16) & (kSramWords - 1);
+        u32 *ram = A.sram_data + st;
+        u32 rv = ram[ra], w0v = ram[wa];
+        ram[wa] = (w0v & ~en) | (wdata & en);
+        wo_v[num_ios - num_srams + s] = rv;
+        if (is_x) {
+          u32 rx = 0;
+          if (A.sram_xmask) {
+            u32 *ramx = A.sram_xmask + st;
+            rx = ramx[ra];
+            u32 w0x = ramx[wa];
+            ramx[wa] = (w0x & ~en) | (wdata_x & en);
+          }
+          wo_x[num_ios - num_srams + s] = rx;
+        }
+      }
+    } else if ((u32)t < nperm) {
+      u32 d = num_ios - num_srams - num_dups + ((u32)t - num_srams * 4);
+      wo_v[d] = pv;
+      if (is_x) wo_x[d] = px;
+    }
+    __syncthreads();
+
+    // ---- clock-enable permutation + commit, kernel_v1_impl.cuh:446-558
+    {
+      uint4 p0 = ld4(script + pi + 4 * t), p1 = ld4(script + pi + 1024 + 4 * t);
+      uint4 p2 = ld4(script + pi + 2048 + 4 * t), p3 = ld4(script + pi + 3072 + 4 * t);
+      uint4 f = ld4(script + pi + 4096 + 4 * t);
+      if ((u32)t < num_ios) {
+        u32 ck = (gather_bits(wo_v, p0, p1, p2, p3) & ~f.y) ^ f.x;
+        u32 data = wo_v[t] ^ f.z;
+        u32 w = io_offset + (u32)t;
+        out[w] = (in[w] & ~ck) | (data & ck);
+        if (is_x) {
+          u32 ckx = gather_bits(wo_x, p0, p1, p2, p3) & ~f.y;
+          u32 ox = in[xoff + w];
+          out[xoff + w] = (ox & ~ck & ~ckx) | (wo_x[t] & ck & ~ckx) | ckx;
+        }
+        if (TIMING) {
+          u16 arr = wo_arr[t];
+          if (A.arrival_state_offset != 0)
+            out[(u32)A.arrival_state_offset + w] = (u32)arr;
+          if (ck != 0 && A.constraints) {
+            u32 cons = A.constraints[w];
+            if (cons != 0) {
+              u32 setup = cons >> 16, hold = cons & 0xffffu;
+              if (arr > 0 && (u32)arr + setup > A.clock_ps)
+                push_event(A.events, A.ev_cap, 4u, w, cycle,
+                           (int)A.clock_ps - (int)arr - (int)setup, arr, setup);
+              if ((u32)arr < hold)
+                push_event(A.events, A.ev_cap, 5u, w, cycle,
+                           (int)arr - (int)hold, arr, hold);
+            }
+          }
+        }
+      }
+    }
+    pi += 5120;
+    if (PROFILE) {
+      __syncthreads();
+      if (t == 0 && cycle < A.sample_cycles && part < A.num_parts)
+        A.samples[(size_t)cycle * A.num_parts + part] = (u32)(clock64() - t_start);
+    }
+    ++part;
+    __syncthreads();
+    if (is_last) break;
+  }
+}
+
+template <bool TIMING, bool PROFILE>
+__global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
+  if (A.timing) {
+    A.clock_ps = A.timing[0];
+    A.constraints = A.timing + 1;
+  }
+  __shared__ u32 st_v[256], st_x[256], hin_v[256], hin_x[256];
+  __shared__ u32 wo_v[256], wo_x[256];
+  __shared__ u16 st_arr[256], wo_arr[256];
+  const int t = threadIdx.x;
+  st_x[t] = 0;
+  hin_x[t] = 0;
+  wo_x[t] = 0;
+  wo_arr[t] = 0;
+  st_arr[t] = 0;
+  u32 gen = read_gen(A.bar);
+  __syncthreads();
+  const u32 b = blockIdx.x;
+  for (u32 c = 0; c < A.num_cycles; ++c) {
+    const u32 *in = A.states + (size_t)c * A.state_size;
+    u32 *out = A.states + (size_t)(c + 1) * A.state_size;
+    for (u32 m = 0; m < A.num_major_stages; ++m) {
+      usize s0 = A.blocks_start[(size_t)m * A.num_blocks + b];
+      u32 pb = PROFILE ? A.part_index[(size_t)m * A.num_blocks + b] : 0u;
+      run_block<TIMING, PROFILE>(A, A.blocks_data + s0, in, out, c, pb, st_v,
+                                 st_x, hin_v, hin_x, wo_v, wo_x, st_arr, wo_arr);
+      grid_sync(A.bar, A.num_blocks, gen);
+    }
+  }
+}
+
+thread_local std::string t_err;
+u32 g_slot = 0;
+
+int fail(const std::string &m) {
+  t_err = m;
+  return -1;
+}
+
+template <bool TIMING, bool PROFILE>
+int launch_scan(ScanArgs &A, hipStream_t stream) {
+  static_assert(sizeof(EvHdr) == 16 && sizeof(Ev) == 32, "event layout");
+  u32 *bar_base = nullptr;
+  if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(g_barrier)) != hipSuccess)
+    return fail("hipGetSymbolAddress(g_barrier) failed");
+  A.bar = bar_base + 2 * (__atomic_fetch_add(&g_slot, 1u, __ATOMIC_RELAXED) % kBarrierSlots);
+  void *args[] = {&A};
+  hipError_t e = hipLaunchCooperativeKernel(
+      reinterpret_cast<const void *>(&scan_kernel<TIMING, PROFILE>),
+      dim3(A.num_blocks), dim3(256), args, 0, stream);
+  if (e != hipSuccess)
+    return fail(std::string("scan launch failed: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int scan_common(usize num_blocks, usize num_major_stages, const usize *blocks_start,
+                const u32 *blocks_data, u32 *sram_data, u32 *sram_xmask,
+                usize num_cycles, usize state_size, u32 *states,
+                const u32 *timing_constraints, u8 *event_buffer, u32 ev_cap,
+                i32 arrival_state_offset, hipStream_t stream,
+                const u32 *part_index, u32 num_parts, u32 *samples,
+                u32 sample_cycles) {
+  t_err.clear();
+  if (num_blocks == 0 || num_blocks > 65535) return fail("num_blocks out of range");
+  if (num_cycles == 0) return 0;
+  if (num_cycles > 0xFFFFFFFFull) return fail("num_cycles out of range");
+  ScanArgs A{};
+  A.num_blocks = (u32)num_blocks;
+  A.num_major_stages = (u32)num_major_stages;
+  A.blocks_start = blocks_start;
+  A.blocks_data = blocks_data;
+  A.sram_data = sram_data;
+  A.sram_xmask = sram_xmask;
+  A.num_cycles = (u32)num_cycles;
+  A.state_size = state_size;
+  A.states = states;
+  A.events = reinterpret_cast<EvHdr *>(event_buffer);
+  A.ev_cap = ev_cap;
+  A.arrival_state_offset = arrival_state_offset;
+  A.timing = timing_constraints;  // [clock_ps, words...], read on device
+  if (A.timing && !A.events) return fail("timing constraints need an event buffer");
+  A.part_index = part_index;
+  A.num_parts = num_parts;
+  A.samples = samples;
+  A.sample_cycles = sample_cycles;
+  const bool timing = A.timing != nullptr || arrival_state_offset != 0;
+  if (samples) return launch_scan<false, true>(A, stream);
+  if (timing) return launch_scan<true, false>(A, stream);
+  return launch_scan<false, false>(A, stream);
+}
+
+void die_on(int rc) {
+  if (rc) {
+    printf("HIP error in jacquard backend: %s\n", t_err.c_str());
+    exit(EXIT_FAILURE);
+  }
+}
+
+}  // namespace
+}  // namespace jqd
+
+using namespace jqd;
+
+extern "C" void simulate_v1_noninteractive_simple_scan_hip(
+    usize num_blocks, usize num_major_stages, const usize *blocks_start,
+    const u32 *blocks_data, u32 *sram_data, u32 *sram_xmask, usize num_cycles,
+    usize state_size, u32 *states_noninteractive, i32 arrival_state_offset) {
+  die_on(scan_common(num_blocks, num_major_stages, blocks_start, blocks_data,
+                     sram_data, sram_xmask, num_cycles, state_size,
+                     states_noninteractive, nullptr, nullptr, 0,
+                     arrival_state_offset, (hipStream_t)0, nullptr, 0, nullptr, 0));
+}
+
+extern "C" void simulate_v1_noninteractive_timed_hip(
+    usize num_blocks, usize num_major_stages, const usize *blocks_start,
+    const u32 *blocks_data, u32 *sram_data, u32 *sram_xmask, usize num_cycles,
+    usize state_size, u32 *states_noninteractive, const u32 *timing_constraints,
+    u8 *event_buffer, i32 arrival_state_offset) {
+  die_on(scan_common(num_blocks, num_major_stages, blocks_start, blocks_data,
+                     sram_data, sram_xmask, num_cycles, state_size,
+                     states_noninteractive, timing_constraints, event_buffer,
+                     1024u, arrival_state_offset, (hipStream_t)0, nullptr, 0,
+                     nullptr, 0));
+}
+
+extern "C" int jq_simulate_scan(usize num_blocks, usize num_major_stages,
+                                const usize *blocks_start, const u32 *blocks_data,
+                                u32 *sram_data, u32 *sram_xmask, usize num_cycles,
+                                usize state_size, u32 *states,
+                                const u32 *timing_constraints, u8 *event_buffer,
+                                u32 event_capacity, i32 arrival_state_offset,
+                                void *stream) {
+  return scan_common(num_blocks, num_major_stages, blocks_start, blocks_data,
+                     sram_data, sram_xmask, num_cycles, state_size, states,
+                     timing_constraints, event_buffer, event_capacity,
+                     arrival_state_offset, (hipStream_t)stream, nullptr, 0,
+                     nullptr, 0);
+}
+
+extern "C" int jq_simulate_scan_profiled(
+    usize num_blocks, usize num_major_stages, const usize *blocks_start,
+    const u32 *blocks_data, u32 *sram_data, u32 *sram_xmask, usize num_cycles,
+    usize state_size, u32 *states, const u32 *part_index, u32 num_parts,
+    u32 *samples, u32 sample_cycles, void *stream) {
+  if (!part_index || !samples) return fail("profiling buffers missing");
+  return scan_common(num_blocks, num_major_stages, blocks_start, blocks_data,
+                     sram_data, sram_xmask, num_cycles, state_size, states,
+                     nullptr, nullptr, 0, 0, (hipStream_t)stream, part_index,
+                     num_parts, samples, sample_cycles);
+}
+
+extern "C" const char *jq_last_error(void) { return t_err.c_str(); }
+
+extern "C" int jq_device_error_flag(u32 *out, int clear) {
+  u32 v = 0;
+  if (hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_error), 4) != hipSuccess) return -1;
+  if (clear) {
+    u32 z = 0;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_error), &z, 4) != hipSuccess) return -1;
+  }
+  *out = v;
+  return 0;
+}

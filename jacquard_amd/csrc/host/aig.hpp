@@ -1,0 +1,137 @@
+// AIG model used by the script compiler.
+//
+// Mirrors the reference's AIG contract (src/aig.rs:41-346): pins are numbered
+// 1..num_pins in topological order, pin 0 is tied to 0, literals are
+// pin<<1|invert, AND gates are structurally hashed, and endpoints are primary
+// outputs, DFFs (D, clock enable, Q) and 13-bit x 32-bit synchronous SRAMs.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace jq {
+
+enum Driver : uint8_t {
+  kTie0 = 0,
+  kAnd = 1,
+  kInput = 2,
+  kClockFlag = 3,
+  kDffQ = 4,
+  kSramQ = 5,
+};
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kSramAddrBits = 13;          // src/aigpdk.rs:20
+constexpr uint32_t kSramWords = 1u << 13;  // src/aigpdk.rs:22
+
+struct Dff {
+  uint32_t d_iv = 0;
+  uint32_t en_iv = 0;
+  uint32_t q = 0;
+};
+
+struct Sram {
+  uint32_t r_addr_iv[kSramAddrBits] = {};
+  uint32_t r_en_iv = 0;
+  uint32_t rd_pin[32] = {};  // 0 = unused bit
+  uint32_t w_addr_iv[kSramAddrBits] = {};
+  uint32_t w_en_iv[32] = {};
+  uint32_t w_data_iv[32] = {};
+};
+
+struct Aig {
+  uint32_t num_pins = 0;
+  std::vector<uint8_t> drv{kTie0};
+  std::vector<uint32_t> a{0}, b{0};  // AND operands (literals)
+  std::vector<uint32_t> ref{0};      // port / dff / sram index of a source
+  std::unordered_map<uint64_t, uint32_t> and_cache;
+
+  std::vector<uint32_t> pos;  // primary outputs (literals), unique
+  std::vector<std::string> po_names;
+  // Port view for VCD I/O, in module declaration order (vector bits from
+  // the declared MSB side): every output bit (name, literal; several bits may
+  // share one PO literal) and every input bit (name, pin, posedge flag pin or
+  // 0 when the port is not a clock).
+  std::vector<std::string> out_port_names;
+  std::vector<uint32_t> out_port_lits;
+  std::vector<std::string> in_port_names;
+  std::vector<uint32_t> in_port_pins, in_port_flag;
+  std::string module_name;
+  std::vector<Dff> dffs;
+  std::vector<Sram> srams;
+  // per-pin integer gate delay in ps (timing), empty when not loaded
+  std::vector<uint16_t> delay_ps;
+
+  uint32_t add_pin(Driver d, uint32_t r = 0) {
+    ++num_pins;
+    drv.push_back(d);
+    a.push_back(0);
+    b.push_back(0);
+    ref.push_back(r);
+    return num_pins;
+  }
+
+  // Structural hashing and constant folding as add_and_gate
+  // (src/aig.rs:327-346).
+  uint32_t add_and(uint32_t x, uint32_t y) {
+    if (x == 0 || y == 0) return 0;
+    if (x == 1) return y;
+    if (y == 1) return x;
+    if (x > y) std::swap(x, y);
+    uint64_t key = (uint64_t)x << 32 | y;
+    auto it = and_cache.find(key);
+    if (it != and_cache.end()) return it->second << 1;
+    uint32_t p = add_pin(kAnd);
+    a[p] = x;
+    b[p] = y;
+    and_cache.emplace(key, p);
+    return p << 1;
+  }
+
+  size_t num_and() const {
+    size_t n = 0;
+    for (auto d : drv) n += (d == kAnd);
+    return n;
+  }
+
+  // endpoint group index space: [POs | DFFs | SRAMs] (flatten.rs:1268)
+  size_t num_endpoints() const { return pos.size() + dffs.size() + srams.size(); }
+
+  // Calls f(pin) for every fan-in pin of endpoint group e
+  // (EndpointGroup::for_each_input, src/aig.rs:134-168).
+  template <class F>
+  void for_each_endpoint_input(size_t e, F f) const {
+    auto g = [&](uint32_t iv) {
+      if ((iv >> 1) >= 1) f(iv >> 1);
+    };
+    if (e < pos.size()) {
+      g(pos[e]);
+      return;
+    }
+    e -= pos.size();
+    if (e < dffs.size()) {
+      g(dffs[e].en_iv);
+      g(dffs[e].d_iv);
+      return;
+    }
+    e -= dffs.size();
+    const Sram &s = srams[e];
+    g(s.r_en_iv);
+    for (int i = 0; i < kSramAddrBits; ++i) {
+      g(s.r_addr_iv[i]);
+      g(s.w_addr_iv[i]);
+    }
+    for (int i = 0; i < 32; ++i) {
+      g(s.w_en_iv[i]);
+      g(s.w_data_iv[i]);
+    }
+  }
+
+  // X-capable pins: DFF Q and SRAM read data are X sources, forward cone,
+  // DFF fixpoint (src/aig.rs:3276-3367).
+  std::vector<uint8_t> x_capable_pins() const;
+};
+
+}  // namespace jq

@@ -1,0 +1,1098 @@
+// Script compiler: level split, partitioning, boomerang mapping, flatten.
+//
+// Format fidelity (what must match the reference bit for bit) is cited per
+// step; the partitioning/mapping policy is this project's own.
+#include "compile.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+namespace jq {
+
+namespace {
+
+constexpr uint32_t kL1 = kWidth / 2;  // 4096 level-1 slots
+
+// ---------------------------------------------------------------------------
+// Levelization and level split (semantics of StagedAIG::from_split,
+// src/staging.rs:93-179: nodes at or below a split that are consumed above it
+// become staged outputs, readable later in the same cycle).
+struct Levels {
+  std::vector<uint32_t> lvl;
+  uint32_t max_lvl = 0;
+};
+
+Levels levelize(const Aig &g) {
+  Levels L;
+  L.lvl.assign(g.num_pins + 1, 0);
+  for (uint32_t p = 1; p <= g.num_pins; ++p)
+    if (g.drv[p] == kAnd) {
+      L.lvl[p] = 1 + std::max(L.lvl[g.a[p] >> 1], L.lvl[g.b[p] >> 1]);
+      L.max_lvl = std::max(L.max_lvl, L.lvl[p]);
+    }
+  return L;
+}
+
+// ---------------------------------------------------------------------------
+// Level-1 list-scheduling boomerang mapper.  Every boomerang stage evaluates
+// one wave of ready AND gates in hier[1] (4096 slots, operands duplicated
+// freely in hier[0]) and carries every still-needed value as a pass-through.
+// Outputs are packed densely at the front of the stage that produces them and
+// written out from there.
+struct MapInput {
+  std::vector<uint32_t> nodes;    // AND pins, ascending (topological)
+  std::vector<uint32_t> inputs;   // available pins
+  std::vector<uint32_t> outputs;  // pins that must be written out
+};
+
+struct Scratch {
+  std::vector<int32_t> loc;  // pin -> local index (or -1)
+  std::vector<uint32_t> stamp;
+  uint32_t cur = 1;
+  explicit Scratch(size_t n) : loc(n, -1), stamp(n, 0) {}
+};
+
+bool map_partition(const Aig &g, const MapInput &in, Scratch &sc,
+                   std::vector<BoomStage> &out, uint32_t max_write_words) {
+  out.clear();
+  // local numbering: inputs first, then nodes
+  std::vector<uint32_t> pins;
+  pins.reserve(in.inputs.size() + in.nodes.size());
+  for (auto p : in.inputs) pins.push_back(p);
+  for (auto p : in.nodes) pins.push_back(p);
+  const uint32_t ni = (uint32_t)in.inputs.size();
+  const uint32_t n = (uint32_t)pins.size();
+  for (uint32_t i = 0; i < n; ++i) sc.loc[pins[i]] = (int32_t)i;
+  struct Guard {
+    Scratch &s;
+    const std::vector<uint32_t> &p;
+    ~Guard() {
+      for (auto x : p) s.loc[x] = -1;
+    }
+  } guard{sc, pins};
+
+  std::vector<uint32_t> rem(n, 0);        // unscheduled consumers
+  std::vector<uint8_t> is_out(n, 0), written(n, 0), sched(n, 0);
+  std::vector<uint8_t> pending(n, 0);     // operands not yet available
+  std::vector<std::vector<uint32_t>> cons(n);
+  std::vector<uint32_t> opa(n, 0), opb(n, 0);
+  for (uint32_t i = ni; i < n; ++i) {
+    uint32_t p = pins[i];
+    int32_t la = sc.loc[g.a[p] >> 1], lb = sc.loc[g.b[p] >> 1];
+    if (la < 0 || lb < 0) return false;  // cone not closed
+    opa[i] = (uint32_t)la;
+    opb[i] = (uint32_t)lb;
+    cons[la].push_back(i);
+    ++rem[la];
+    if (lb != la) {
+      cons[lb].push_back(i);
+      ++rem[lb];
+    }
+  }
+  for (auto p : in.outputs) {
+    int32_t l = sc.loc[p];
+    if (l < 0) return false;
+    is_out[l] = 1;
+  }
+  // height (longest path to an output) as priority
+  std::vector<uint32_t> height(n, 0);
+  for (uint32_t i = n; i-- > ni;) {
+    for (auto c : cons[i]) height[i] = std::max(height[i], height[c] + 1);
+    (void)0;
+  }
+  std::vector<uint8_t> avail(n, 0);
+  std::vector<uint32_t> avail_list;
+  for (uint32_t i = 0; i < ni; ++i) {
+    avail[i] = 1;
+    avail_list.push_back(i);
+  }
+  for (uint32_t i = ni; i < n; ++i) pending[i] = (uint8_t)(opa[i] == opb[i] ? 1 : 2);
+  std::vector<uint32_t> ready;
+  for (uint32_t i = ni; i < n; ++i) {
+    // operands that are inputs are available from the start
+    uint8_t k = 0;
+    if (opa[i] >= ni) ++k;
+    if (opb[i] >= ni && opb[i] != opa[i]) ++k;
+    pending[i] = k;
+    if (k == 0) ready.push_back(i);
+  }
+  uint32_t remaining = n - ni;
+  uint32_t total_words = 0;
+  std::vector<uint32_t> taken(n, 0);  // consumers scheduled this stage
+  while (true) {
+    bool any_unwritten = false;
+    for (auto v : avail_list)
+      if (is_out[v] && !written[v]) {
+        any_unwritten = true;
+        break;
+      }
+    if (remaining == 0 && !any_unwritten) break;
+    if (out.size() > 4096) return false;
+    // select all ready nodes, then defer low-priority ones while over capacity
+    std::sort(ready.begin(), ready.end(), [&](uint32_t x, uint32_t y) {
+      if (height[x] != height[y]) return height[x] > height[y];
+      return x < y;
+    });
+    std::vector<uint32_t> S = ready;
+    for (auto x : S) {
+      ++taken[opa[x]];
+      if (opb[x] != opa[x]) ++taken[opb[x]];
+    }
+    auto carried = [&](uint32_t v) {
+      return rem[v] > taken[v] || (is_out[v] && !written[v]);
+    };
+    uint32_t ncarry = 0;
+    for (auto v : avail_list) ncarry += carried(v);
+    // defer from the tail (lowest priority)
+    size_t keep = S.size();
+    for (size_t k = S.size(); k-- > 0 && keep + ncarry > kL1;) {
+      uint32_t x = S[k];
+      // deferring frees one slot; operands that stop being fully consumed
+      // become carries
+      int delta = -1;
+      uint32_t ops[2] = {opa[x], opb[x]};
+      int nops = opa[x] == opb[x] ? 1 : 2;
+      for (int t = 0; t < nops; ++t)
+        if (!carried(ops[t]) && taken[ops[t]] == rem[ops[t]]) ++delta;
+      if (delta >= 0) continue;
+      for (int t = 0; t < nops; ++t) --taken[ops[t]];
+      S[k] = kNone;
+      --keep;
+      ncarry += (uint32_t)(delta + 1);
+    }
+    if (keep + ncarry > kL1) {
+      for (auto x : S)
+        if (x != kNone) {
+          taken[opa[x]] = 0;
+          taken[opb[x]] = 0;
+        }
+      return false;
+    }
+    S.erase(std::remove(S.begin(), S.end(), kNone), S.end());
+    // layout: outputs produced/needed now first, then the rest
+    std::vector<uint32_t> layout;
+    layout.reserve(keep + ncarry);
+    for (auto x : S)
+      if (is_out[x]) layout.push_back(x);
+    for (auto v : avail_list)
+      if (is_out[v] && !written[v]) layout.push_back(v);
+    uint32_t nout = (uint32_t)layout.size();
+    for (auto x : S)
+      if (!is_out[x]) layout.push_back(x);
+    for (auto v : avail_list)
+      if (carried(v) && !(is_out[v] && !written[v])) layout.push_back(v);
+    for (auto x : S) {
+      taken[opa[x]] = 0;
+      taken[opb[x]] = 0;
+    }
+    BoomStage bs;
+    bs.hier[0].assign(kWidth, kNone);
+    bs.hier[1].assign(kL1, kNone);
+    for (uint32_t j = 0; j < layout.size(); ++j) {
+      uint32_t v = layout[j];
+      if (v >= ni && !avail[v]) {  // computed here
+        bs.hier[0][j] = g.a[pins[v]] >> 1;
+        bs.hier[0][j + kL1] = g.b[pins[v]] >> 1;
+      } else {
+        bs.hier[0][j] = pins[v];
+      }
+      bs.hier[1][j] = pins[v];
+    }
+    uint32_t nw = (nout + 31) / 32;
+    for (uint32_t w = 0; w < nw; ++w) bs.write_outs.push_back((uint16_t)(128 + w));
+    total_words += nw;
+    if (total_words > max_write_words) return false;
+    out.push_back(std::move(bs));
+    // advance state
+    for (auto x : S) {
+      sched[x] = 1;
+      --remaining;
+      --rem[opa[x]];
+      if (opb[x] != opa[x]) --rem[opb[x]];
+    }
+    for (uint32_t j = 0; j < nout; ++j) written[layout[j]] = 1;
+    for (auto v : avail_list) avail[v] = 0;
+    avail_list.assign(layout.begin(), layout.end());
+    for (auto v : avail_list) avail[v] = 1;
+    std::vector<uint32_t> next_ready;
+    for (auto r : ready)
+      if (!sched[r]) next_ready.push_back(r);
+    for (auto x : S)
+      for (auto c : cons[x])
+        if (--pending[c] == 0) next_ready.push_back(c);
+    ready.swap(next_ready);
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Endpoint bookkeeping for one major stage.
+struct EndpointInfo {
+  std::vector<uint32_t> outs;  // pins that must be written out
+  uint32_t nsram = 0;
+};
+
+void endpoint_outputs(const Aig &g, const MajorStage &ms, size_t e,
+                      EndpointInfo &info) {
+  info.outs.clear();
+  info.nsram = 0;
+  if (e < ms.staged_out.size()) {
+    info.outs.push_back(ms.staged_out[e]);
+    return;
+  }
+  size_t oe = ms.orig_endpoints[e - ms.staged_out.size()];
+  if (oe >= g.pos.size() + g.dffs.size()) info.nsram = 1;
+  g.for_each_endpoint_input(oe, [&](uint32_t p) { info.outs.push_back(p); });
+  if (oe >= g.pos.size() && oe < g.pos.size() + g.dffs.size()) {
+    const Dff &d = g.dffs[oe - g.pos.size()];
+    if (d.d_iv <= 1) info.outs.clear();  // constant-D DFF needs no slot
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
+                        std::string *log) {
+  std::ostringstream lg;
+  Levels L = levelize(g);
+  uint32_t split = opt.level_split ? opt.level_split : std::max<uint32_t>(1, L.max_lvl);
+  uint32_t nmajor = std::max<uint32_t>(1, (L.max_lvl + split - 1) / split);
+  auto stage_of = [&](uint32_t p) -> int {
+    if (g.drv[p] != kAnd) return -1;
+    return (int)std::min<uint32_t>((L.lvl[p] - 1) / split, nmajor - 1);
+  };
+  // endpoint stage = deepest fan-in stage (sources -> 0)
+  size_t ne = g.num_endpoints();
+  std::vector<int> ep_stage(ne, 0);
+  for (size_t e = 0; e < ne; ++e) {
+    int s = 0;
+    g.for_each_endpoint_input(e, [&](uint32_t p) { s = std::max(s, stage_of(p)); });
+    ep_stage[e] = s;
+  }
+  // last stage in which each node is consumed
+  std::vector<int> last_use(g.num_pins + 1, -1);
+  for (uint32_t p = 1; p <= g.num_pins; ++p)
+    if (g.drv[p] == kAnd) {
+      int s = stage_of(p);
+      last_use[g.a[p] >> 1] = std::max(last_use[g.a[p] >> 1], s);
+      last_use[g.b[p] >> 1] = std::max(last_use[g.b[p] >> 1], s);
+    }
+  for (size_t e = 0; e < ne; ++e)
+    g.for_each_endpoint_input(e, [&](uint32_t p) {
+      last_use[p] = std::max(last_use[p], ep_stage[e]);
+    });
+  out.stages.assign(nmajor, MajorStage());
+  for (uint32_t p = 1; p <= g.num_pins; ++p) {
+    int s = stage_of(p);
+    if (s >= 0 && last_use[p] > s) out.stages[s].staged_out.push_back(p);
+  }
+  for (size_t e = 0; e < ne; ++e) out.stages[ep_stage[e]].orig_endpoints.push_back((uint32_t)e);
+
+  Scratch sc(g.num_pins + 1);
+  std::vector<uint32_t> mark_all(g.num_pins + 1, 0);
+  std::vector<uint32_t> seen(g.num_pins + 1, 0);
+  uint32_t gstamp = 1, sstamp = 1;
+  std::vector<uint32_t> lvl_m(g.num_pins + 1, 0);
+  size_t total_parts = 0, total_bstages = 0, total_nodes = 0;
+
+  for (uint32_t m = 0; m < nmajor; ++m) {
+    MajorStage &ms = out.stages[m];
+    auto avail_pin = [&](uint32_t p) { return stage_of(p) < (int)m; };
+    // stage-relative levels (cone depth above this stage's inputs)
+    // endpoints: staged outputs then original endpoints
+    size_t nep = ms.num_endpoints();
+    std::vector<EndpointInfo> infos(nep);
+    for (size_t e = 0; e < nep; ++e) endpoint_outputs(g, ms, e, infos[e]);
+
+    // greedy grouping of endpoints; a group's cone is tracked with stamps
+    std::vector<uint32_t> stack;
+    std::vector<uint32_t> cur_nodes, cur_inputs;
+    std::vector<uint32_t> lvl_count;
+    uint32_t cur_outs = 0, cur_srams = 0;
+    std::vector<uint32_t> add_nodes, add_inputs;
+    std::vector<uint32_t> &mark = mark_all;  // group membership stamp
+    auto collect = [&](size_t e) {
+      add_nodes.clear();
+      add_inputs.clear();
+      ++sstamp;
+      for (auto p : infos[e].outs) stack.push_back(p);
+      while (!stack.empty()) {
+        uint32_t p = stack.back();
+        stack.pop_back();
+        if (p == 0 || seen[p] == sstamp || mark[p] == gstamp) continue;
+        seen[p] = sstamp;
+        if (avail_pin(p)) {
+          add_inputs.push_back(p);
+          continue;
+        }
+        add_nodes.push_back(p);
+        stack.push_back(g.a[p] >> 1);
+        stack.push_back(g.b[p] >> 1);
+      }
+    };
+    // stage-relative level of every node of this stage
+    for (uint32_t p = 1; p <= g.num_pins; ++p) {
+      if (g.drv[p] != kAnd || stage_of(p) != (int)m) continue;
+      uint32_t la = avail_pin(g.a[p] >> 1) ? 0 : lvl_m[g.a[p] >> 1];
+      uint32_t lb = avail_pin(g.b[p] >> 1) ? 0 : lvl_m[g.b[p] >> 1];
+      lvl_m[p] = 1 + std::max(la, lb);
+    }
+    std::vector<std::vector<uint32_t>> group_eps;
+    std::vector<uint32_t> ge;
+    auto flush = [&]() {
+      if (!ge.empty()) group_eps.push_back(ge);
+      ge.clear();
+      ++gstamp;
+      cur_nodes.clear();
+      cur_inputs.clear();
+      lvl_count.clear();
+      cur_outs = 0;
+      cur_srams = 0;
+    };
+    for (size_t e = 0; e < nep; ++e) {
+      collect(e);
+      uint32_t nouts = (uint32_t)infos[e].outs.size();
+      bool fits = true;
+      if (!ge.empty()) {
+        if (cur_inputs.size() + add_inputs.size() > opt.max_inputs) fits = false;
+        if (cur_outs + nouts > opt.max_outputs) fits = false;
+        if (cur_nodes.size() + add_nodes.size() > opt.max_part_nodes) fits = false;
+        if (cur_srams + infos[e].nsram > 48) fits = false;
+        if (fits) {
+          std::map<uint32_t, uint32_t> extra;
+          for (auto p : add_nodes) ++extra[lvl_m[p]];
+          for (auto &kv : extra) {
+            uint32_t have = kv.first < lvl_count.size() ? lvl_count[kv.first] : 0;
+            if (have + kv.second > opt.max_level_nodes) fits = false;
+          }
+        }
+        if (!fits) {
+          flush();
+          collect(e);
+        }
+      }
+      ge.push_back((uint32_t)e);
+      for (auto p : add_nodes) {
+        mark[p] = gstamp;
+        cur_nodes.push_back(p);
+        if (lvl_m[p] >= lvl_count.size()) lvl_count.resize(lvl_m[p] + 1, 0);
+        ++lvl_count[lvl_m[p]];
+      }
+      for (auto p : add_inputs) {
+        mark[p] = gstamp;
+        cur_inputs.push_back(p);
+      }
+      cur_outs += nouts;
+      cur_srams += infos[e].nsram;
+    }
+    flush();
+
+    // map every group; split in halves on failure
+    std::function<void(std::vector<uint32_t>)> build = [&](std::vector<uint32_t> eps) {
+      MapInput mi;
+      ++sstamp;
+      uint32_t nsram = 0;
+      for (auto e : eps) {
+        nsram += infos[e].nsram;
+        for (auto p : infos[e].outs) {
+          if (p == 0) continue;
+          if (seen[p] != sstamp) {
+            seen[p] = sstamp;
+            mi.outputs.push_back(p);
+            stack.push_back(p);
+          }
+        }
+      }
+      // cone closure
+      ++gstamp;
+      std::vector<uint32_t> outs_copy = mi.outputs;
+      for (auto p : outs_copy) stack.push_back(p);
+      while (!stack.empty()) {
+        uint32_t p = stack.back();
+        stack.pop_back();
+        if (p == 0 || mark[p] == gstamp) continue;
+        mark[p] = gstamp;
+        if (avail_pin(p)) {
+          mi.inputs.push_back(p);
+          continue;
+        }
+        mi.nodes.push_back(p);
+        stack.push_back(g.a[p] >> 1);
+        stack.push_back(g.b[p] >> 1);
+      }
+      std::sort(mi.nodes.begin(), mi.nodes.end());
+      std::sort(mi.inputs.begin(), mi.inputs.end());
+      // duplicate words: distinct activations of one pin beyond the first
+      std::map<uint32_t, std::vector<uint32_t>> act;
+      for (auto e : eps) {
+        if (e < ms.staged_out.size()) {
+          auto &v = act[ms.staged_out[e]];
+          if (std::find(v.begin(), v.end(), 2u) == v.end()) v.push_back(2u);
+          continue;
+        }
+        size_t oe = ms.orig_endpoints[e - ms.staged_out.size()];
+        uint32_t pin = 0, key = 0;
+        if (oe < g.pos.size()) {
+          if (g.pos[oe] <= 1) continue;
+          pin = g.pos[oe] >> 1;
+          key = 2 | (g.pos[oe] & 1);
+        } else if (oe < g.pos.size() + g.dffs.size()) {
+          const Dff &d = g.dffs[oe - g.pos.size()];
+          if (d.d_iv <= 1) continue;
+          pin = d.d_iv >> 1;
+          key = d.en_iv << 1 | (d.d_iv & 1);
+        } else {
+          continue;
+        }
+        auto &v = act[pin];
+        if (std::find(v.begin(), v.end(), key) == v.end()) v.push_back(key);
+      }
+      uint32_t ndup = 0;
+      for (auto &kv : act) ndup += (uint32_t)kv.second.size() - 1;
+      uint32_t dup_words = (ndup + 31) / 32;
+      bool ok = mi.inputs.size() <= kWidth && nsram * 4 + dup_words <= 256 &&
+                nsram + dup_words < 256;
+      std::vector<BoomStage> stages;
+      if (ok)
+        ok = map_partition(g, mi, sc, stages, 256 - nsram - dup_words);
+      if (!ok) {
+        if (eps.size() <= 1)
+          throw std::runtime_error("compile: a single endpoint does not fit a partition");
+        std::vector<uint32_t> a(eps.begin(), eps.begin() + eps.size() / 2);
+        std::vector<uint32_t> b(eps.begin() + eps.size() / 2, eps.end());
+        build(a);
+        build(b);
+        return;
+      }
+      Partition part;
+      part.endpoints = eps;
+      part.stages = std::move(stages);
+      total_bstages += part.stages.size();
+      total_nodes += mi.nodes.size();
+      ms.parts.push_back(std::move(part));
+    };
+    for (auto &eps : group_eps) build(eps);
+    // sort by decreasing number of boomerang stages (src/pe.rs:1068)
+    std::stable_sort(ms.parts.begin(), ms.parts.end(),
+                     [](const Partition &x, const Partition &y) {
+                       return x.stages.size() > y.stages.size();
+                     });
+    total_parts += ms.parts.size();
+    lg << "major stage " << m << ": " << ms.staged_out.size() << " staged outputs, "
+       << ms.orig_endpoints.size() << " endpoints, " << ms.parts.size()
+       << " partitions\n";
+  }
+  lg << "total: " << total_parts << " partitions, " << total_bstages
+     << " boomerang stages, " << total_nodes << " mapped AND evaluations ("
+     << g.num_and() << " unique)\n";
+  if (log) *log = lg.str();
+}
+
+// ---------------------------------------------------------------------------
+// Flatten (src/flatten.rs:470-1130 for one partition, 1133-1573 overall).
+namespace {
+
+struct PermFlags {
+  std::vector<uint16_t> perm;  // 8192
+  std::vector<uint32_t> inv, set0, extra;
+  void init() {
+    perm.assign(kWidth, 0);
+    inv.assign(kThreads, 0);
+    set0.assign(kThreads, 0xFFFFFFFFu);
+    extra.assign(kThreads, 0);
+  }
+};
+
+inline void set_bit(std::vector<uint32_t> &v, uint32_t pos, uint32_t bit) {
+  if (bit)
+    v[pos >> 5] |= 1u << (pos & 31);
+  else
+    v[pos >> 5] &= ~(1u << (pos & 31));
+}
+
+struct FlatPart {
+  uint32_t num_srams = 0, num_normal = 0, num_dups = 0, num_writeouts = 0;
+  uint32_t state_start = 0, sram_start = 0;
+  std::vector<uint32_t> wo_pins;  // normal writeout bits (num_normal*32)
+  std::unordered_map<uint32_t, uint32_t> pin2pos;
+  // pin -> ordered (activation key, placed position or kNone)
+  std::unordered_map<uint32_t, std::vector<std::pair<uint32_t, uint32_t>>> acts;
+  uint32_t cnt_dup = 0;
+  PermFlags sd, ck;  // sram/dup permutation, clock-enable permutation
+};
+
+// after-layout of one boomerang stage: index (8192>>h)+j for level h >= 1
+inline uint32_t after_at(const BoomStage &s, uint32_t idx) {
+  if (idx == 0) return kNone;
+  int h = kBoomStages;
+  while ((kWidth >> h) <= idx && h > 0) --h;  // level with base 8192>>h <= idx
+  // base = 8192 >> h ... find h with base <= idx < 2*base
+  h = 0;
+  for (int lv = 1; lv <= kBoomStages; ++lv) {
+    uint32_t base = kWidth >> lv;
+    if (idx >= base && idx < 2 * base) {
+      h = lv;
+      break;
+    }
+  }
+  return s.at(h, idx - (kWidth >> h));
+}
+
+uint32_t query_perm(const FlatPart &fp, uint32_t lit, uint32_t &perm,
+                    uint32_t &inv, uint32_t &set0) {
+  if (lit <= 1) {
+    perm = 0;
+    inv = lit;
+    set0 = 1;
+    return 0;
+  }
+  auto it = fp.pin2pos.find(lit >> 1);
+  if (it == fp.pin2pos.end()) throw std::runtime_error("flatten: pin not written out");
+  perm = it->second;
+  inv = lit & 1;
+  set0 = 0;
+  return 0;
+}
+
+void place(PermFlags &pf, uint32_t pos, uint32_t perm, uint32_t inv, uint32_t set0) {
+  pf.perm[pos] = (uint16_t)perm;
+  set_bit(pf.inv, pos, inv);
+  set_bit(pf.set0, pos, set0);
+}
+
+void place_ck(FlatPart &fp, uint32_t pos, uint32_t clken_lit, uint32_t data_inv) {
+  uint32_t perm, inv, set0;
+  query_perm(fp, clken_lit, perm, inv, set0);
+  place(fp.ck, pos, perm, inv, set0);
+  set_bit(fp.ck.extra, pos, data_inv);
+}
+
+// get_or_place_output_with_activation (src/flatten.rs:639-692)
+uint32_t get_or_place(FlatPart &fp, uint32_t lit, uint32_t clken_lit) {
+  auto &v = fp.acts.at(lit >> 1);
+  uint32_t key = clken_lit << 1 | (lit & 1);
+  size_t idx = 0;
+  while (idx < v.size() && v[idx].first != key) ++idx;
+  if (idx == v.size()) throw std::runtime_error("flatten: activation missing");
+  if (v[idx].second != kNone) return v[idx].second;
+  auto it = fp.pin2pos.find(lit >> 1);
+  if (it == fp.pin2pos.end()) throw std::runtime_error("flatten: output not placed");
+  uint32_t origpos = it->second, r;
+  if (idx == 0) {
+    place_ck(fp, origpos, clken_lit, lit & 1);
+    r = origpos;
+  } else {
+    ++fp.cnt_dup;
+    uint32_t dup_pos = (fp.num_writeouts - fp.num_srams) * 32 - fp.cnt_dup;
+    uint32_t dup_perm_pos = (fp.num_srams * 4 + fp.num_dups) * 32 - fp.cnt_dup;
+    if (dup_perm_pos >= kWidth) throw std::runtime_error("flatten: duplicate overflow");
+    place(fp.sd, dup_perm_pos, origpos, 0, 0);
+    place_ck(fp, dup_pos, clken_lit, lit & 1);
+    r = dup_pos;
+  }
+  v[idx].second = r;
+  return r;
+}
+
+// Pack (word, mask) global reads into 256 threads x rounds, <= 32 bits per
+// thread (contract of map_global_read_to_rounds, src/flatten.rs:343-400).
+void pack_global_reads(std::vector<std::pair<uint32_t, uint32_t>> entries,
+                       std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &th) {
+  th.assign(kThreads, {});
+  std::vector<uint32_t> bits(kThreads, 0);
+  std::stable_sort(entries.begin(), entries.end(), [](auto &x, auto &y) {
+    return __builtin_popcount(x.second) > __builtin_popcount(y.second);
+  });
+  std::vector<std::pair<uint32_t, uint32_t>> work(entries.rbegin(), entries.rend());
+  while (!work.empty()) {
+    auto e = work.back();
+    work.pop_back();
+    uint32_t pc = (uint32_t)__builtin_popcount(e.second);
+    int best = -1;
+    for (uint32_t t = 0; t < kThreads; ++t) {
+      if (bits[t] + pc > 32) continue;
+      if (best < 0 || th[t].size() < th[best].size() ||
+          (th[t].size() == th[best].size() && bits[t] > bits[best]))
+        best = (int)t;
+    }
+    if (best < 0) {
+      // split the mask in two halves
+      uint32_t m = e.second, lo = 0;
+      for (uint32_t k = 0; k < pc / 2; ++k) {
+        uint32_t low = m & (0u - m);
+        lo |= low;
+        m ^= low;
+      }
+      if (!lo) throw std::runtime_error("flatten: global read does not fit");
+      work.push_back({e.first, m});
+      work.push_back({e.first, lo});
+      continue;
+    }
+    th[best].push_back(e);
+    bits[best] += pc;
+  }
+}
+
+}  // namespace
+
+void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
+             const std::vector<std::vector<double>> *costs, Script &s) {
+  s = Script();
+  const uint32_t nb = opt.num_blocks;
+  s.num_blocks = nb;
+  s.num_major_stages = (uint32_t)c.stages.size();
+  for (uint32_t p = 1; p <= g.num_pins; ++p)
+    if (g.drv[p] == kInput || g.drv[p] == kClockFlag) s.input_layout.push_back(p);
+  for (uint32_t i = 0; i < s.input_layout.size(); ++i) s.input_map[s.input_layout[i]] = i;
+  const uint32_t const_zero_pos = (uint32_t)s.input_layout.size();
+  uint32_t states_start = (uint32_t)((s.input_layout.size() + 31) / 32);
+  if (const_zero_pos == states_start * 32) states_start += 1;
+  uint32_t sum_state = states_start, sum_sram = 0;
+  s.sram_offsets.assign(g.srams.size(), kNone);
+  s.dff_data_pos.assign(g.dffs.size(), kNone);
+
+  std::vector<std::vector<std::vector<uint32_t>>> blocks_parts(c.stages.size());
+  std::vector<std::vector<FlatPart>> fparts(c.stages.size());
+
+  for (size_t m = 0; m < c.stages.size(); ++m) {
+    const MajorStage &ms = c.stages[m];
+    auto &bp = blocks_parts[m];
+    bp.assign(nb, {});
+    // masonry (src/flatten.rs:1207-1226), cost per partition
+    std::vector<double> tot(nb, 0.0);
+    auto cost_of = [&](size_t i) {
+      if (costs && m < costs->size() && i < (*costs)[m].size()) return (*costs)[m][i];
+      return (double)ms.parts[i].stages.size() + 3.0;
+    };
+    std::vector<size_t> order(ms.parts.size());
+    std::iota(order.begin(), order.end(), 0);
+    if (costs)
+      std::stable_sort(order.begin(), order.end(),
+                       [&](size_t x, size_t y) { return cost_of(x) > cost_of(y); });
+    for (size_t k = 0; k < order.size(); ++k) {
+      size_t i = order[k];
+      size_t put;
+      if (k < nb) {
+        put = k;
+      } else {
+        put = (size_t)(std::min_element(tot.begin(), tot.end()) - tot.begin());
+      }
+      bp[put].push_back((uint32_t)i);
+      tot[put] += cost_of(i);
+    }
+    auto &fps = fparts[m];
+    fps.assign(ms.parts.size(), FlatPart());
+    // init_afters_writeouts (src/flatten.rs:470-586)
+    for (size_t i = 0; i < ms.parts.size(); ++i) {
+      const Partition &pt = ms.parts[i];
+      FlatPart &fp = fps[i];
+      for (const BoomStage &bs : pt.stages)
+        for (uint16_t wo : bs.write_outs)
+          for (uint32_t k = 0; k < 32; ++k) fp.wo_pins.push_back(after_at(bs, wo * 32u + k));
+      fp.num_normal = 0;
+      for (const BoomStage &bs : pt.stages) fp.num_normal += (uint32_t)bs.write_outs.size();
+      uint32_t ndupbits = 0;
+      for (uint32_t e : pt.endpoints) {
+        if (e < ms.staged_out.size()) {
+          auto &v = fp.acts[ms.staged_out[e]];
+          bool have = false;
+          for (auto &kv : v) have |= kv.first == 2u;
+          if (!have) v.push_back({2u, kNone});
+          continue;
+        }
+        size_t oe = ms.orig_endpoints[e - ms.staged_out.size()];
+        uint32_t pin, key;
+        if (oe < g.pos.size()) {
+          if (g.pos[oe] <= 1) continue;
+          pin = g.pos[oe] >> 1;
+          key = 2 | (g.pos[oe] & 1);
+        } else if (oe < g.pos.size() + g.dffs.size()) {
+          const Dff &d = g.dffs[oe - g.pos.size()];
+          if (d.d_iv <= 1) continue;
+          pin = d.d_iv >> 1;
+          key = d.en_iv << 1 | (d.d_iv & 1);
+        } else {
+          ++fp.num_srams;
+          continue;
+        }
+        auto &v = fp.acts[pin];
+        bool have = false;
+        for (auto &kv : v) have |= kv.first == key;
+        if (!have) v.push_back({key, kNone});
+      }
+      for (auto &kv : fp.acts) ndupbits += (uint32_t)kv.second.size() - 1;
+      fp.num_dups = (ndupbits + 31) / 32;
+      fp.num_writeouts = fp.num_normal + fp.num_srams + fp.num_dups;
+      if (fp.num_writeouts > 256 || fp.num_srams * 4 + fp.num_dups > 256)
+        throw std::runtime_error("flatten: partition exceeds 256 writeout words");
+      for (uint32_t k = 0; k < fp.wo_pins.size(); ++k)
+        if (fp.wo_pins[k] != kNone) fp.pin2pos[fp.wo_pins[k]] = k;
+    }
+    // state and SRAM allocation in block-affinity order
+    // (src/flatten.rs:1247-1278)
+    for (auto &blk : bp)
+      for (uint32_t pid : blk) {
+        FlatPart &fp = fps[pid];
+        fp.state_start = sum_state;
+        sum_state += fp.num_writeouts;
+        fp.sram_start = sum_sram;
+        sum_sram += fp.num_srams * kSramWords;
+      }
+    // make_inputs_outputs (src/flatten.rs:694-858)
+    for (size_t i = 0; i < ms.parts.size(); ++i) {
+      const Partition &pt = ms.parts[i];
+      FlatPart &fp = fps[i];
+      fp.sd.init();
+      fp.ck.init();
+      fp.cnt_dup = 0;
+      uint32_t cur_sram = 0;
+      for (uint32_t e : pt.endpoints) {
+        if (e < ms.staged_out.size()) {
+          uint32_t p = ms.staged_out[e];
+          s.staged_io_map[p] = fp.state_start * 32 + get_or_place(fp, p << 1, 1);
+          continue;
+        }
+        size_t oe = ms.orig_endpoints[e - ms.staged_out.size()];
+        if (oe < g.pos.size()) {
+          uint32_t lit = g.pos[oe];
+          if (lit <= 1) {
+            s.output_map[lit] = 0;
+            continue;
+          }
+          s.output_map[lit] = fp.state_start * 32 + get_or_place(fp, lit, 1);
+        } else if (oe < g.pos.size() + g.dffs.size()) {
+          size_t di = oe - g.pos.size();
+          const Dff &d = g.dffs[di];
+          if (d.d_iv <= 1) {
+            s.input_map[d.q] = const_zero_pos;
+            continue;
+          }
+          uint32_t pos = fp.state_start * 32 + get_or_place(fp, d.d_iv, d.en_iv);
+          s.output_map[d.d_iv] = pos;
+          s.input_map[d.q] = pos;
+          s.dff_data_pos[di] = pos;
+        } else {
+          size_t si = oe - g.pos.size() - g.dffs.size();
+          const Sram &r = g.srams[si];
+          uint32_t rd_local = fp.num_writeouts - fp.num_srams + cur_sram;
+          uint32_t rd_global = fp.state_start + rd_local;
+          s.sram_offsets[si] = fp.sram_start + cur_sram * kSramWords;
+          for (uint32_t k = 0; k < 32; ++k) {
+            uint32_t d = r.rd_pin[k];
+            if (!d) continue;
+            s.input_map[d] = rd_global * 32 + k;
+            s.output_map[d << 1] = rd_global * 32 + k;
+            place_ck(fp, rd_local * 32 + k, r.r_en_iv, 0);
+          }
+          uint32_t st = cur_sram * 128, perm, inv, set0;
+          for (int k = 0; k < kSramAddrBits; ++k) {
+            query_perm(fp, r.r_addr_iv[k], perm, inv, set0);
+            place(fp.sd, st + k, perm, inv, set0);
+            query_perm(fp, r.w_addr_iv[k], perm, inv, set0);
+            place(fp.sd, st + 16 + k, perm, inv, set0);
+          }
+          for (int k = 0; k < 32; ++k) {
+            query_perm(fp, r.w_en_iv[k], perm, inv, set0);
+            place(fp.sd, st + 32 + k, perm, inv, set0);
+            query_perm(fp, r.w_data_iv[k], perm, inv, set0);
+            place(fp.sd, st + 64 + k, perm, inv, set0);
+          }
+          ++cur_sram;
+        }
+      }
+      if ((fp.cnt_dup + 31) / 32 != fp.num_dups)
+        throw std::runtime_error("flatten: duplicate count mismatch");
+    }
+  }
+  s.reg_io_state_size = sum_state;
+  s.sram_storage_size = sum_sram;
+
+  // X-capability per partition (reference rule src/flatten.rs:1424-1498 plus
+  // two closures that keep the template X of non-X partitions out of every
+  // X-capable read and out of primary outputs).
+  std::vector<std::vector<uint8_t>> px(c.stages.size());
+  if (opt.xprop) {
+    std::vector<uint8_t> xp = g.x_capable_pins();
+    // word ranges and reads per partition
+    struct PI_ {
+      uint32_t m, i, lo, hi;
+      std::vector<uint32_t> reads;  // state words read
+      bool po;
+    };
+    std::vector<PI_> all;
+    std::vector<int32_t> word_owner(sum_state, -1);
+    for (size_t m = 0; m < c.stages.size(); ++m) {
+      px[m].assign(c.stages[m].parts.size(), 0);
+      for (size_t i = 0; i < c.stages[m].parts.size(); ++i) {
+        const Partition &pt = c.stages[m].parts[i];
+        const FlatPart &fp = fparts[m][i];
+        PI_ r{(uint32_t)m, (uint32_t)i, fp.state_start,
+              fp.state_start + fp.num_writeouts, {}, false};
+        bool x = false;
+        for (const BoomStage &bs : pt.stages)
+          for (int h = 0; h <= kBoomStages && !x; ++h)
+            for (uint32_t pin : bs.hier[h])
+              if (pin != kNone && xp[pin]) {
+                x = true;
+                break;
+              }
+        for (uint32_t pin : pt.stages[0].hier[0]) {
+          if (pin == kNone) continue;
+          auto it = s.input_map.find(pin);
+          uint32_t pos = it != s.input_map.end() ? it->second : s.staged_io_map.at(pin);
+          r.reads.push_back(pos >> 5);
+        }
+        for (uint32_t e : pt.endpoints)
+          if (e >= c.stages[m].staged_out.size() &&
+              c.stages[m].orig_endpoints[e - c.stages[m].staged_out.size()] < g.pos.size())
+            r.po = true;
+        px[m][i] = x || r.po;
+        for (uint32_t w = r.lo; w < r.hi; ++w) word_owner[w] = (int32_t)all.size();
+        all.push_back(std::move(r));
+      }
+    }
+    for (bool changed = true; changed;) {
+      changed = false;
+      for (auto &r : all) {
+        for (uint32_t w : r.reads) {
+          int32_t o = w < word_owner.size() ? word_owner[w] : -1;
+          if (o < 0) continue;
+          auto &ro = all[o];
+          uint8_t &me = px[r.m][r.i], &them = px[ro.m][ro.i];
+          if (them && !me) {
+            me = 1;
+            changed = true;
+          }
+          if (me && !them) {
+            them = 1;
+            changed = true;
+          }
+        }
+      }
+    }
+    s.xprop = true;
+    s.xprop_state_offset = sum_state;
+  }
+
+  // build_script per partition (src/flatten.rs:863-1130) and assemble
+  for (size_t m = 0; m < c.stages.size(); ++m) {
+    const MajorStage &ms = c.stages[m];
+    std::vector<std::vector<uint32_t>> scripts(ms.parts.size());
+    std::vector<std::vector<std::pair<uint64_t, std::vector<uint32_t>>>> patches(ms.parts.size());
+    for (size_t i = 0; i < ms.parts.size(); ++i) {
+      const Partition &pt = ms.parts[i];
+      const FlatPart &fp = fparts[m][i];
+      std::vector<uint32_t> &sc = scripts[i];
+      sc.reserve(256 + 5120 * (pt.stages.size() + 2) + 1024);
+      sc.push_back((uint32_t)pt.stages.size());
+      sc.push_back(0);
+      sc.push_back(fp.num_writeouts);
+      sc.push_back(fp.state_start);
+      sc.push_back(fp.num_srams);
+      sc.push_back(fp.sram_start);
+      sc.push_back(0);
+      sc.push_back(fp.num_dups);
+      sc.push_back(opt.xprop ? (uint32_t)px[m][i] : 0u);
+      sc.push_back(opt.xprop ? s.xprop_state_offset : 0u);
+      while (sc.size() < 128) sc.push_back(0);
+      uint32_t last = kNone;
+      for (size_t j = 0; j < pt.stages.size(); ++j)
+        for (uint16_t wo : pt.stages[j].write_outs) {
+          uint32_t cw = (uint32_t)j << 8 | wo;
+          if (last == kNone) {
+            last = cw;
+          } else {
+            sc.push_back(last | cw << 16);
+            last = kNone;
+          }
+        }
+      if (last != kNone) sc.push_back(last | 0xFFFFu << 16);
+      while (sc.size() < 256) sc.push_back(0xFFFFFFFFu);
+      if (sc.size() != 256) throw std::runtime_error("flatten: too many writeout hooks");
+      // global reads
+      std::map<uint32_t, uint32_t> taken;  // word|flag -> mask
+      for (uint32_t pin : pt.stages[0].hier[0]) {
+        if (pin == kNone) continue;
+        auto it = s.input_map.find(pin);
+        if (it != s.input_map.end()) {
+          taken[it->second >> 5] |= 1u << (it->second & 31);
+        } else {
+          uint32_t pos = s.staged_io_map.at(pin);
+          taken[(pos >> 5) | 0x80000000u] |= 1u << (pos & 31);
+        }
+      }
+      std::vector<std::pair<uint32_t, uint32_t>> ent(taken.begin(), taken.end());
+      std::vector<std::vector<std::pair<uint32_t, uint32_t>>> th;
+      pack_global_reads(ent, th);
+      uint32_t rounds = 0;
+      for (auto &v : th) rounds = std::max(rounds, (uint32_t)v.size());
+      sc[6] = rounds;
+      size_t gstart = sc.size();
+      sc.resize(gstart + 2 * (size_t)rounds * kThreads, 0);
+      std::unordered_map<uint64_t, uint16_t> local;  // (flag,pos) -> local bit
+      for (uint32_t t = 0; t < kThreads; ++t) {
+        for (size_t r = 0; r < th[t].size(); ++r) {
+          sc[gstart + kThreads * 2 * r + 2 * t] = th[t][r].first;
+          sc[gstart + kThreads * 2 * r + 2 * t + 1] = th[t][r].second;
+        }
+        uint32_t bit_id = 0;
+        for (size_t r = th[t].size(); r-- > 0;) {
+          uint32_t idx = th[t][r].first, mask = th[t][r].second;
+          uint64_t flag = idx >> 31;
+          for (int k = 31; k >= 0; --k)
+            if (mask >> k & 1) {
+              uint64_t key = flag << 40 | (uint64_t)((idx & 0x7FFFFFFFu) * 32ull + k);
+              local[key] = (uint16_t)(t * 32 + bit_id);
+              ++bit_id;
+            }
+        }
+      }
+      std::vector<uint32_t> prev_pos;  // pin -> position map of previous stage
+      std::unordered_map<uint32_t, uint16_t> last_pos;
+      for (uint32_t pin : pt.stages[0].hier[0]) {
+        if (pin == kNone || last_pos.count(pin)) continue;
+        auto it = s.input_map.find(pin);
+        uint64_t key;
+        if (it != s.input_map.end())
+          key = (uint64_t)it->second;
+        else
+          key = 1ull << 40 | s.staged_io_map.at(pin);
+        last_pos[pin] = local.at(key);
+      }
+      for (size_t j = 0; j < pt.stages.size(); ++j) {
+        const BoomStage &bs = pt.stages[j];
+        std::vector<uint16_t> perm(kWidth, 0);
+        for (uint32_t i2 = 0; i2 < kWidth; ++i2) {
+          uint32_t pin = bs.hier[0][i2];
+          if (pin != kNone) perm[i2] = last_pos.at(pin);
+        }
+        std::vector<uint32_t> xa(kThreads, 0), xb(kThreads, 0), ob(kThreads, 0);
+        std::vector<std::vector<uint32_t>> tpins(kThreads);
+        for (int h = 1; h <= kBoomStages; ++h) {
+          if (bs.hier[h].empty()) continue;
+          uint32_t len = kWidth >> h;
+          for (uint32_t q = 0; q < len; ++q) {
+            uint32_t o = bs.hier[h][q];
+            if (o == kNone) continue;
+            uint32_t pa = bs.at(h - 1, q), pb = bs.at(h - 1, q + len);
+            uint32_t pos = len + q;
+            tpins[pos >> 5].push_back(o);
+            if (o == pa) {
+              ob[pos >> 5] |= 1u << (pos & 31);
+              continue;
+            }
+            if (g.drv[o] != kAnd) throw std::runtime_error("flatten: bad tree node");
+            uint32_t la = g.a[o], lb = g.b[o];
+            if ((la >> 1) == pa && (lb >> 1) == pb) {
+            } else if ((lb >> 1) == pa && (la >> 1) == pb) {
+              std::swap(la, lb);
+            } else {
+              throw std::runtime_error("flatten: tree operands misplaced");
+            }
+            if (la & 1) xa[pos >> 5] |= 1u << (pos & 31);
+            if (lb & 1) xb[pos >> 5] |= 1u << (pos & 31);
+          }
+        }
+        for (uint32_t k = 0; k < 4; ++k)
+          for (uint32_t t = 0; t < kThreads; ++t) {
+            uint32_t i2 = t * 32 + k * 8;
+            for (uint32_t q = 0; q < 8; q += 2)
+              sc.push_back((uint32_t)perm[i2 + q] | (uint32_t)perm[i2 + q + 1] << 16);
+          }
+        for (uint32_t t = 0; t < kThreads; ++t) {
+          sc.push_back(xa[t]);
+          sc.push_back(xb[t]);
+          sc.push_back(ob[t]);
+          if (!tpins[t].empty()) patches[i].push_back({sc.size(), tpins[t]});
+          sc.push_back(0);
+        }
+        last_pos.clear();
+        for (int h = kBoomStages; h >= 1; --h) {
+          if (bs.hier[h].empty()) continue;
+          uint32_t len = kWidth >> h;
+          for (uint32_t q = 0; q < len; ++q) {
+            uint32_t o = bs.hier[h][q];
+            if (o != kNone) last_pos[o] = (uint16_t)(len + q);
+          }
+        }
+      }
+      auto emit = [&](const PermFlags &pf, bool third) {
+        for (uint32_t k = 0; k < 4; ++k)
+          for (uint32_t t = 0; t < kThreads; ++t) {
+            uint32_t i2 = t * 32 + k * 8;
+            for (uint32_t q = 0; q < 8; q += 2)
+              sc.push_back((uint32_t)pf.perm[i2 + q] | (uint32_t)pf.perm[i2 + q + 1] << 16);
+          }
+        for (uint32_t t = 0; t < kThreads; ++t) {
+          sc.push_back(pf.inv[t]);
+          sc.push_back(pf.set0[t]);
+          sc.push_back(third ? pf.extra[t] : 0);
+          sc.push_back(0);
+        }
+      };
+      emit(fp.sd, false);
+      emit(fp.ck, true);
+    }
+    for (uint32_t blk = 0; blk < nb; ++blk) {
+      s.blocks_start.push_back(s.blocks_data.size());
+      const auto &lst = blocks_parts[m][blk];
+      if (lst.empty()) {
+        std::vector<uint32_t> dummy(kThreads, 0);
+        dummy[1] = 1;
+        s.blocks_data.insert(s.blocks_data.end(), dummy.begin(), dummy.end());
+        continue;
+      }
+      uint64_t last_st = 0;
+      for (size_t k = 0; k < lst.size(); ++k) {
+        uint32_t pid = lst[k];
+        uint64_t base = s.blocks_data.size();
+        if (k + 1 == lst.size()) last_st = base;
+        s.blocks_data.insert(s.blocks_data.end(), scripts[pid].begin(), scripts[pid].end());
+        for (auto &pp : patches[pid]) s.delay_patch.push_back({base + pp.first, pp.second});
+        s.parts.push_back({(uint32_t)m, blk, pid, base, (uint64_t)scripts[pid].size()});
+        if (opt.xprop) s.part_x.push_back(px[m][pid]);
+      }
+      s.blocks_data[last_st + 1] = 1;
+    }
+    std::vector<std::vector<uint32_t>>().swap(scripts);
+  }
+  s.blocks_start.push_back(s.blocks_data.size());
+  s.blocks_data.resize(s.blocks_data.size() + kThreads * 8, 0);
+  (void)c;
+}
+
+void inject_timing(const Aig &g, Script &s) {
+  if (g.delay_ps.empty()) return;
+  for (auto &pp : s.delay_patch) {
+    uint32_t mx = 0;
+    for (uint32_t pin : pp.second)
+      if (pin < g.delay_ps.size()) mx = std::max<uint32_t>(mx, g.delay_ps[pin]);
+    s.blocks_data[pp.first] = std::min<uint32_t>(mx, 65535);
+  }
+  s.timing = true;
+}
+
+std::vector<uint32_t> constraint_buffer(const Script &s, uint32_t clock_ps,
+                                        uint16_t setup_ps, uint16_t hold_ps) {
+  std::vector<uint32_t> buf(1 + s.reg_io_state_size, 0);
+  buf[0] = clock_ps;
+  for (uint32_t pos : s.dff_data_pos) {
+    if (pos == kNone) continue;
+    uint32_t w = pos / 32;
+    if (w >= s.reg_io_state_size) continue;
+    uint32_t ex = buf[1 + w];
+    uint16_t os = (uint16_t)(ex >> 16), oh = (uint16_t)(ex & 0xFFFF);
+    uint16_t ns = os == 0 ? setup_ps : std::min(os, setup_ps);
+    uint16_t nh = oh == 0 ? hold_ps : std::min(oh, hold_ps);
+    buf[1 + w] = (uint32_t)ns << 16 | nh;
+  }
+  return buf;
+}
+
+}  // namespace jq

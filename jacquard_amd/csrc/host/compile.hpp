@@ -1,0 +1,114 @@
+// Script compiler: AIG -> major stages -> partitions -> boomerang stages ->
+// FlattenedScriptV1 (the packed program the device interpreter runs).
+//
+// The output format is the reference's script ISA bit for bit
+// (src/flatten.rs:198-258, 863-1130, 1133-1573); the partitioning and
+// boomerang mapping algorithms are this project's own (any valid mapping
+// yields the same simulated values).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "aig.hpp"
+
+namespace jq {
+
+constexpr int kBoomStages = 13;                  // src/pe.rs:27
+constexpr uint32_t kWidth = 1u << kBoomStages;   // 8192 bits
+constexpr uint32_t kThreads = kWidth >> 5;       // 256 words
+
+// One boomerang stage: hier[0] (8192 inputs) ... hier[13] (1 output).  A
+// level left empty is all-unused.  write_outs lists after-layout words
+// (0..255) whose values are written out after this stage.
+struct BoomStage {
+  std::vector<uint32_t> hier[kBoomStages + 1];
+  std::vector<uint16_t> write_outs;
+  uint32_t at(int h, uint32_t j) const {
+    return hier[h].empty() ? kNone : hier[h][j];
+  }
+};
+
+struct Partition {
+  std::vector<uint32_t> endpoints;  // indices into MajorStage endpoint space
+  std::vector<BoomStage> stages;
+};
+
+// One level-split slice of the AIG (mirrors StagedAIG, src/staging.rs:12-21).
+// Endpoint space: [staged_out pins | original endpoint groups].
+struct MajorStage {
+  std::vector<uint32_t> staged_out;  // pins produced here, read later
+  std::vector<uint32_t> orig_endpoints;
+  std::vector<Partition> parts;
+  size_t num_endpoints() const { return staged_out.size() + orig_endpoints.size(); }
+};
+
+struct CompileOptions {
+  uint32_t num_blocks = 256;
+  // level-split period in logic levels (0 = a single major stage)
+  uint32_t level_split = 0;
+  bool xprop = false;
+  bool timing = false;
+  // partition sizing knobs (soft limits used by the greedy grouping)
+  uint32_t max_inputs = 6144;
+  uint32_t max_outputs = 2048;
+  uint32_t max_level_nodes = 2600;
+  uint32_t max_part_nodes = 24000;
+};
+
+struct Script {
+  uint32_t num_blocks = 0, num_major_stages = 0;
+  std::vector<uint64_t> blocks_start;
+  std::vector<uint32_t> blocks_data;
+  uint32_t reg_io_state_size = 0;
+  uint32_t sram_storage_size = 0;  // u32 entries
+  std::vector<uint32_t> input_layout;
+  std::unordered_map<uint32_t, uint32_t> input_map;      // pin -> bit
+  std::unordered_map<uint32_t, uint32_t> output_map;     // literal -> bit
+  std::unordered_map<uint32_t, uint32_t> staged_io_map;  // pin -> bit
+  std::vector<uint32_t> sram_offsets;                    // per SRAM index
+  bool xprop = false;
+  uint32_t xprop_state_offset = 0;
+  std::vector<uint8_t> part_x;  // flat over stages/blocks/parts
+  // timing
+  bool timing = false;
+  std::vector<std::pair<uint64_t, std::vector<uint32_t>>> delay_patch;
+  std::vector<uint32_t> dff_data_pos;  // per DFF: state bit of its D (or ~0)
+  // per-partition records for cost models: stage, block, offset, length
+  struct PartRec {
+    uint32_t major, block, index_in_stage;
+    uint64_t offset, length;
+  };
+  std::vector<PartRec> parts;
+};
+
+// Full pipeline state kept between the partitioning and flatten steps so
+// block assignment can be redone with another cost model.
+struct Compiled {
+  std::vector<MajorStage> stages;
+  std::vector<std::vector<uint8_t>> part_x;  // per stage, per partition
+};
+
+// Level-split + partition + map.  Throws std::runtime_error on failure.
+void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
+                        std::string *log = nullptr);
+
+// Block assignment ("masonry", src/flatten.rs:1207-1226) with per-partition
+// costs (nullptr = the reference heuristic stages+3) and flatten into the
+// script format.  costs is indexed [major stage][partition].
+void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
+             const std::vector<std::vector<double>> *costs, Script &s);
+
+// Fill delay slots from per-pin delays (inject_timing_to_script,
+// src/flatten.rs:1812-1846).
+void inject_timing(const Aig &g, Script &s);
+
+// Per-state-word constraint words (build_timing_constraint_buffer,
+// src/flatten.rs:1891-1924).  Returns [clock_ps, words...].
+std::vector<uint32_t> constraint_buffer(const Script &s, uint32_t clock_ps,
+                                        uint16_t setup_ps, uint16_t hold_ps);
+
+}  // namespace jq

@@ -1,0 +1,242 @@
+"""FlattenedScriptV1: the packed program of the Boolean-processor interpreter.
+
+Mirrors the reference type (src/flatten.rs:189-341) field for field:
+``blocks_start`` / ``blocks_data`` in the exact script ISA of
+src/flatten.rs:198-258, the state layout (primary-input words first, then
+partition writeouts in block-affinity order, X masks at
+``xprop_state_offset``, arrivals at ``arrival_state_offset``), and the
+input/output maps.  Construction goes through the native compiler
+(level split -> partitions -> boomerang stages -> flatten).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from .netlist import Netlist
+
+SRAM_WORDS = 1 << 13  # src/aigpdk.rs:22
+THREADS = 256
+
+
+class Compiled:
+    """Partitioned + mapped design, flattenable for any block count and
+    per-partition cost vector."""
+
+    def __init__(self, netlist: Netlist, level_split=0, max_inputs=0,
+                 max_outputs=0, max_level_nodes=0, max_part_nodes=0):
+        self.netlist = netlist
+        h = N.host().jqh_compile(netlist.handle, level_split, max_inputs,
+                                 max_outputs, max_level_nodes, max_part_nodes)
+        if not h:
+            raise RuntimeError("compile failed: " + N.host_error())
+        self._h = h
+        self.log = N.host().jqh_last_log().decode()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.host().jqh_compiled_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def num_major_stages(self):
+        return N.host().jqh_compiled_num_major(self._h)
+
+    def parts_per_stage(self):
+        L = N.host()
+        return [L.jqh_compiled_num_parts(self._h, m)
+                for m in range(self.num_major_stages())]
+
+    def stage_counts(self):
+        """Boomerang stages per partition, [major][partition]."""
+        L = N.host()
+        return [[L.jqh_compiled_part_stages(self._h, m, p)
+                 for p in range(n)]
+                for m, n in enumerate(self.parts_per_stage())]
+
+    def flatten(self, num_blocks, xprop=False, costs=None):
+        return FlattenedScriptV1(self, num_blocks, xprop=xprop, costs=costs)
+
+
+class FlattenedScriptV1:
+    def __init__(self, compiled: Compiled, num_blocks, xprop=False, costs=None):
+        self.compiled = compiled
+        self.netlist = compiled.netlist
+        L = N.host()
+        cptr = None
+        if costs is not None:
+            flat = np.ascontiguousarray(np.concatenate(
+                [np.asarray(c, np.float64) for c in costs]), np.float64)
+            self._costs = flat
+            cptr = flat.ctypes.data
+        h = L.jqh_flatten(self.netlist.handle, compiled.handle, num_blocks,
+                          int(bool(xprop)), cptr)
+        if not h:
+            raise RuntimeError("flatten failed: " + N.host_error())
+        self._h = h
+        info = (ctypes.c_uint64 * 15)()
+        L.jqh_script_info(h, info)
+        self.num_blocks = int(info[0])
+        self.num_major_stages = int(info[1])
+        self.reg_io_state_size = int(info[2])
+        self.sram_storage_size = int(info[3])
+        self.xprop_enabled = bool(info[6])
+        self.xprop_state_offset = int(info[7]) if self.xprop_enabled else 0
+        self.timing_enabled = bool(info[13])
+        self.timing_arrivals_enabled = False
+        self.arrival_state_offset = 0
+        self.clock_period_ps = 1000
+        nbs, nbd = int(info[4]), int(info[5])
+        self.blocks_start = np.ctypeslib.as_array(
+            ctypes.cast(L.jqh_script_blocks_start(h), ctypes.POINTER(ctypes.c_uint64)),
+            (nbs,)).copy()
+        self._bd_ptr = L.jqh_script_blocks_data(h)
+        self.blocks_data = np.ctypeslib.as_array(
+            ctypes.cast(self._bd_ptr, ctypes.POINTER(ctypes.c_uint32)), (nbd,))
+        nl = int(info[8])
+        self.input_layout = np.ctypeslib.as_array(
+            ctypes.cast(L.jqh_script_input_layout(h), ctypes.POINTER(ctypes.c_uint32)),
+            (nl,)).copy()
+        ptrs = [ctypes.c_void_p() for _ in range(6)]
+        L.jqh_script_maps(h, *[ctypes.byref(p) for p in ptrs])
+
+        def arr(p, n):
+            if n == 0:
+                return np.zeros(0, np.uint32)
+            return np.ctypeslib.as_array(
+                ctypes.cast(p, ctypes.POINTER(ctypes.c_uint32)), (n,)).copy()
+
+        nin, nout, nst = int(info[9]), int(info[10]), int(info[11])
+        self.input_map = dict(zip(arr(ptrs[0], nin).tolist(), arr(ptrs[1], nin).tolist()))
+        self.output_map = dict(zip(arr(ptrs[2], nout).tolist(), arr(ptrs[3], nout).tolist()))
+        self.staged_io_map = dict(zip(arr(ptrs[4], nst).tolist(), arr(ptrs[5], nst).tolist()))
+        self.num_parts = int(info[12])
+        self.partition_x_capable = (
+            arr(L.jqh_script_part_x(h), self.num_parts).astype(bool)
+            if self.xprop_enabled else np.zeros(0, bool))
+        if self.num_parts:
+            self.parts = np.ctypeslib.as_array(
+                ctypes.cast(L.jqh_script_parts(h), ctypes.POINTER(ctypes.c_uint64)),
+                (self.num_parts, 5)).copy()
+        else:
+            self.parts = np.zeros((0, 5), np.uint64)
+        nsr = int(info[14])
+        self.sram_offsets = arr(L.jqh_script_sram_offsets(h), nsr)
+        nd = self.netlist.counts()["num_dffs"]
+        self.dff_data_pos = arr(L.jqh_script_dff_data_pos(h), nd)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.host().jqh_script_free(h)
+            self._h = None
+
+    # ------------------------------------------------------------------
+    @property
+    def num_srams(self):
+        return self.sram_storage_size // SRAM_WORDS
+
+    def effective_state_size(self):
+        """src/flatten.rs:1581-1590."""
+        size = self.reg_io_state_size
+        if self.xprop_enabled:
+            size += self.reg_io_state_size
+        if self.timing_arrivals_enabled:
+            size += self.reg_io_state_size
+        return size
+
+    def inject_timing(self):
+        """Fill the per-word delay slots from the netlist's gate delays
+        (src/flatten.rs:1812-1846)."""
+        if N.host().jqh_inject_timing(self.netlist.handle, self._h) != 0:
+            raise RuntimeError(N.host_error())
+        self.timing_enabled = True
+
+    def enable_timing_arrivals(self):
+        """src/flatten.rs:1596-1603."""
+        if not self.timing_enabled:
+            raise RuntimeError("Cannot enable timing arrivals without SDF timing data")
+        self.timing_arrivals_enabled = True
+        self.arrival_state_offset = self.reg_io_state_size * (1 + int(self.xprop_enabled))
+
+    def build_timing_constraint_buffer(self, clock_period_ps, setup_ps, hold_ps):
+        """[clock_ps, per-word (min setup << 16 | min hold)...]
+        (src/flatten.rs:1891-1924, src/sim/setup.rs:480-497)."""
+        out = np.zeros(1 + self.reg_io_state_size, np.uint32)
+        N.host().jqh_constraint_buffer(self._h, min(int(clock_period_ps), 0xFFFFFFFF),
+                                       setup_ps, hold_ps, out.ctypes.data)
+        self.clock_period_ps = int(clock_period_ps)
+        return out
+
+    def validate(self, state_size=None):
+        """Check every access the interpreter will make (raises on error)."""
+        ss = state_size if state_size is not None else self.effective_state_size()
+        bs = np.ascontiguousarray(self.blocks_start, np.uint64)
+        rc = N.host().jqh_validate(
+            bs.ctypes.data, bs.size, self._bd_ptr, self.blocks_data.size,
+            self.num_blocks, self.num_major_stages, ss,
+            max(self.sram_storage_size, 0), 0, int(self.arrival_state_offset))
+        if rc != 0:
+            raise ValueError(N.host_error())
+
+    def part_features(self):
+        """24 script features per partition, in blocks_data order."""
+        out = np.zeros((self.num_parts, 24), np.float32)
+        parts = np.ascontiguousarray(self.parts, np.uint64)
+        rc = N.host().jqh_part_features(self._bd_ptr, self.blocks_data.size,
+                                        parts.ctypes.data, self.num_parts,
+                                        out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(N.host_error())
+        return out
+
+    def part_index(self):
+        """Index of the first partition of every (major stage, block) in
+        blocks_data order (the profiled kernel's sample slot)."""
+        idx = np.zeros(self.num_major_stages * self.num_blocks, np.uint32)
+        parts = self.parts
+        k = 0
+        for m in range(self.num_major_stages):
+            for b in range(self.num_blocks):
+                idx[m * self.num_blocks + b] = k
+                while k < len(parts) and parts[k, 0] == m and parts[k, 1] == b:
+                    k += 1
+        return idx
+
+    def xmask_template(self):
+        """X mask of the state layout: 1 everywhere except input_map bits
+        (src/sim/vcd_io.rs:470-474)."""
+        t = np.full(self.reg_io_state_size, 0xFFFFFFFF, np.uint32)
+        pos = np.fromiter(self.input_map.values(), np.uint64, len(self.input_map))
+        np.bitwise_and.at(t, (pos >> 5).astype(np.int64),
+                          ~(np.uint32(1) << (pos & 31).astype(np.uint32)))
+        return t
+
+    def algorithmic_bytes_per_cycle(self):
+        """Algorithmic HBM bytes of one lane-cycle, from the script alone:
+        script words + gathered state words + writeout read/write + SRAM
+        accesses (+X lane, +timing words)."""
+        bd = self.blocks_data
+        total_script = int(self.blocks_start[-1])
+        g = 0
+        io = 0
+        sr = 0
+        for off, length in self.parts[:, 3:5]:
+            h = bd[int(off):int(off) + 256]
+            R = int(h[6])
+            x = 2 if (self.xprop_enabled and h[8]) else 1
+            gr = bd[int(off) + 256:int(off) + 256 + 512 * R].reshape(R, 256, 2)
+            g += x * int(np.count_nonzero(gr[:, :, 1]))
+            io += x * 2 * int(h[2])
+            sr += x * 3 * int(h[4])
+        return 4 * (total_script + g + io + sr)
+
+
+def compile_design(netlist, num_blocks, level_split=0, xprop=False, **kw):
+    """Convenience: partition + flatten with the heuristic cost."""
+    c = Compiled(netlist, level_split=level_split, **kw)
+    return c.flatten(num_blocks, xprop=xprop)

@@ -1,0 +1,103 @@
+"""Parity of the plain drop-in kernel (through the C ABI) with the CPU oracle.
+
+Every snapshot of every cycle must be bit-identical to the oracle's
+kernel-semantics run; the final values are also checked against the direct
+AIG simulation, which does not look at the script at all."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import CONFIG1, design, direct_check
+from jacquard_amd import sim, stimulus
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("num_blocks,split", [(1, 0), (8, 0), (16, 6), (64, 4)])
+def test_config1_2state(num_blocks, split):
+    nl, comp = design("c1", level_split=split, **CONFIG1)
+    s = comp.flatten(num_blocks)
+    cycles = 300
+    _say("config1", num_blocks, split, "parts", s.num_parts)
+    st = stimulus.build_states(s, cycles, seed=11, p=0.5)
+    ref = st.copy()
+    oracle.run(s, ref, cycles)
+    got = sim.run_scan(s, st.copy(), cycles)["states"]
+    assert np.array_equal(got, ref)
+    direct_check(s, got, cycles, 11, 0.5)
+
+
+def test_reference_symbol_entry_point():
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    s = comp.flatten(16)
+    cycles = 100
+    st = stimulus.build_states(s, cycles, seed=12, p=0.3)
+    ref = st.copy()
+    oracle.run(s, ref, cycles)
+    got = sim.run_scan(s, st.copy(), cycles, reference_symbols=True)["states"]
+    assert np.array_equal(got, ref)
+
+
+def test_srams_and_major_stages():
+    nl, comp = design("sram", level_split=8, seed=4, gates=30000, depth=48,
+                      dffs=3000, inputs=64, outputs=32, geometric_p=0.3, srams=4)
+    s = comp.flatten(32)
+    cycles = 120
+    _say("sram", s.num_parts, s.num_major_stages, s.sram_storage_size)
+    st = stimulus.build_states(s, cycles, seed=13, p=0.2)
+    ref = st.copy()
+    sram_ref = np.zeros(s.sram_storage_size, np.uint32)
+    oracle.run(s, ref, cycles, sram=sram_ref)
+    out = sim.run_scan(s, st.copy(), cycles)
+    assert np.array_equal(out["states"], ref)
+    assert np.array_equal(out["sram"][:s.sram_storage_size], sram_ref)
+    direct_check(s, out["states"], cycles, 13, 0.2)
+
+
+@pytest.mark.parametrize("frac", [0.0, 0.1])
+def test_xprop(frac):
+    nl, comp = design("xp", level_split=8, seed=5, gates=20000, depth=40,
+                      dffs=2000, inputs=64, outputs=32, geometric_p=0.3, srams=2)
+    s = comp.flatten(16, xprop=True)
+    cycles = 80
+    m, pick = stimulus.dff_initial_xmask(s, frac, seed=7)
+    st = stimulus.build_states(s, cycles, seed=14, p=0.4, initial_xmask=m)
+    ref = st.copy()
+    sram = np.zeros(s.sram_storage_size, np.uint32)
+    sram_x = np.full(s.sram_storage_size, 0xFFFFFFFF, np.uint32)
+    oracle.run(s, ref, cycles, sram=sram, sram_x=sram_x)
+    out = sim.run_scan(s, st.copy(), cycles)
+    assert np.array_equal(out["states"], ref)
+    assert np.array_equal(out["sram_x"][:s.sram_storage_size], sram_x)
+
+
+def test_timing_events():
+    nl, comp = design("tm", level_split=6, seed=6, gates=8000, depth=30,
+                      dffs=800, inputs=32, outputs=16, geometric_p=0.3)
+    d = np.zeros(nl.counts()["num_pins"] + 1, np.uint16)
+    rng = np.random.default_rng(1)
+    A = nl.arrays()
+    ands = A["drv"] == 1
+    d[ands] = rng.integers(20, 201, ands.sum())
+    nl.set_delays(d)
+    s = comp.flatten(16)
+    s.inject_timing()
+    s.enable_timing_arrivals()
+    cons = s.build_timing_constraint_buffer(190, 50, 20)
+    cycles = 60
+    st = stimulus.build_states(s, cycles, seed=15, p=0.5)
+    ref = st.copy()
+    ev_ref, n_ref = oracle.run(s, ref, cycles, constraints=cons,
+                               arrival_state_offset=s.arrival_state_offset,
+                               event_capacity=1 << 20)
+    out = sim.run_scan(s, st.copy(), cycles, constraints=cons,
+                       event_capacity=1 << 20)
+    assert np.array_equal(out["states"], ref)
+    assert out["event_count"] == n_ref and n_ref > 0
+    a = sim.canonical_events(out["events"])
+    b = sim.canonical_events(ev_ref)
+    assert np.array_equal(a, b)
