@@ -82,31 +82,33 @@ int jq_simulate_scan_profiled(usize num_blocks, usize num_major_stages,
                               u32 *samples, u32 sample_cycles, void *stream);
 
 /* ---- Batched lane-sliced form (no counterpart in the reference) ---------
- * Simulates 32*num_groups independent stimulus lanes of one script in a
- * single persistent launch.  State words are stored bit-transposed: for
- * every state bit, one u32 whose bit l belongs to lane l of the group.
- *   blocks_per_group : script blocks (== the script's num_blocks); the
- *                      launch uses num_groups * blocks_per_group workgroups
- *   group_cycles[g]  : cycles to run for group g
- *   states           : num_groups * 2 * (reg_io_state_size*32) u32 ping-pong
- *                      buffers (lane-words), snapshot 0 in buffer 0
+ * Simulates 32*num_groups independent stimulus lanes of one script (value
+ * lane: X masks and arrivals are not computed in this form) in a single
+ * persistent launch.  State words are stored bit-transposed: for every
+ * state bit, one u32 "lane-word" whose bit l belongs to lane l of a group.
+ *   blocks_per_group : the script's num_blocks; the launch uses
+ *                      num_groups * blocks_per_group co-resident workgroups
+ *   group_cycles[g]  : cycles group g runs (<= max_cycles)
+ *   states           : num_groups * 2 * (reg_io_state_size*32) u32: two
+ *                      ping-pong snapshots per group, snapshot 0 in the
+ *                      first (clock-flag lane-words pre-set to ~0 in both)
  *   sram_data        : num_groups * 32 * sram_storage_size u32 (per lane)
- *   num_input_bits   : primary-input bits (the first bits of the state)
- *   lane_seed/lane_thresh : per-lane stimulus RNG seed and toggle threshold
- *                      (toggle when hash < thresh); 32*num_groups entries
- *   clock_flag_bits  : positions set to 1 every cycle (posedge flags), n_flags
- *   obs_bits/num_obs : state bits captured after every cycle into
- *                      obs_out[g][cycle][k] (lane-words)
- * Returns 0 or a negative error code. */
+ *   data_bits[n_data]: state bit of each toggling primary input
+ *   lane_seed/lane_thresh : per lane (32*num_groups): input bit k toggles at
+ *                      cycle c when mix32(mix32(seed + 0x9E3779B9*k) ^ c) <
+ *                      thresh, mix32 = the "lowbias32" mixer
+ *   obs_bits[num_obs]: state bits captured after every cycle into
+ *                      obs_out[(g*max_cycles + cycle)*num_obs + k]
+ * Returns 0 or a negative error code (message from jq_last_error()). */
 int jq_simulate_batched(usize num_groups, usize blocks_per_group,
                         usize num_major_stages, const usize *blocks_start,
                         const u32 *blocks_data, u32 reg_io_state_size,
                         u32 sram_storage_size, u32 *sram_data,
                         const u32 *group_cycles, u32 max_cycles, u32 *states,
-                        u32 num_input_bits, const u32 *lane_seed,
-                        const u32 *lane_thresh, const u32 *clock_flag_bits,
-                        u32 n_flags, const u32 *obs_bits, u32 num_obs,
-                        u32 *obs_out, void *stream);
+                        u32 n_pi_words, const u32 *data_bits, u32 n_data,
+                        const u32 *lane_seed, const u32 *lane_thresh,
+                        const u32 *obs_bits, u32 num_obs, u32 *obs_out,
+                        void *stream);
 
 /* Occupancy query used by the autotuner: the largest number of resident
  * workgroups (all co-resident, as the persistent launches require) for the

@@ -14,18 +14,24 @@ constexpr int kThreads = 256;      // words per 8192-bit boomerang layer
 constexpr u32 kSramWords = 8192;   // one AIGPDK SRAM (src/aigpdk.rs:22)
 constexpr int kBarrierSlots = 256; // concurrent persistent launches
 
-// Barrier slots (count, generation) and a sticky error word.  Every launch
-// takes its own slot; generations only grow, counts return to zero after
-// every barrier, so no per-launch reset is needed.
-extern __device__ u32 g_barrier[kBarrierSlots * 2];
-extern __device__ u32 g_error;
+// Each kernel source owns its barrier slots (count, generation pairs) and a
+// sticky error word as file-scope device arrays.  Every launch takes its own
+// slot; generations only grow and counts return to zero after every
+// barrier, so no per-launch reset is needed.
+
+// Host side: records the message returned by jq_last_error() for this
+// thread; returns -1 (or 0 for an empty message).
+int set_error(const char *msg);
+
+// Host side: reads (and optionally clears) the batched kernels' error word.
+int batched_error_flag(u32 *out, int clear);
 
 // Grid-wide barrier among `n` co-resident workgroups.  Thread 0 of each
 // workgroup arrives with a device-scope release and waits with a
 // device-scope acquire (these also write back / invalidate the per-XCD L2
 // state that other workgroups read).  The wait is bounded: after ~2^27
 // polls it raises g_error and returns so the grid always drains.
-__device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen) {
+__device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen, u32 *err) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -40,7 +46,7 @@ __device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen) {
                                __HIP_MEMORY_SCOPE_AGENT) == gen) {
         __builtin_amdgcn_s_sleep(1);
         if (++spins > (1u << 27)) {
-          atomicOr(&g_error, 1u);
+          atomicOr(err, 1u);
           break;
         }
       }

@@ -414,7 +414,7 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
       u32 pb = PROFILE ? A.part_index[(size_t)m * A.num_blocks + b] : 0u;
       run_block<TIMING, PROFILE>(A, A.blocks_data + s0, in, out, c, pb, st_v,
                                  st_x, hin_v, hin_x, wo_v, wo_x, st_arr, wo_arr);
-      grid_sync(A.bar, A.num_blocks, gen);
+      grid_sync(A.bar, A.num_blocks, gen, &g_error);
     }
   }
 }
@@ -426,6 +426,15 @@ int fail(const std::string &m) {
   t_err = m;
   return -1;
 }
+
+}  // namespace
+
+int set_error(const char *msg) {
+  t_err = msg;
+  return msg[0] ? -1 : 0;
+}
+
+namespace {
 
 template <bool TIMING, bool PROFILE>
 int launch_scan(ScanArgs &A, hipStream_t stream) {
@@ -541,13 +550,36 @@ extern "C" int jq_simulate_scan_profiled(
 
 extern "C" const char *jq_last_error(void) { return t_err.c_str(); }
 
+namespace jqd {
+// occupancy of the batched kernel, defined next to it
+int batched_blocks_per_cu(int *out);
+}  // namespace jqd
+
+extern "C" int jq_max_resident_blocks(int kind, int *out_blocks) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail("hipGetDevice failed");
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return fail("CU count query failed");
+  if (kind == 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, reinterpret_cast<const void *>(&scan_kernel<true, false>), 256, 0) !=
+        hipSuccess)
+      return fail("occupancy query failed");
+  } else {
+    if (batched_blocks_per_cu(&per)) return -1;
+  }
+  *out_blocks = per * cus;
+  return 0;
+}
+
 extern "C" int jq_device_error_flag(u32 *out, int clear) {
-  u32 v = 0;
+  u32 v = 0, vb = 0;
   if (hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_error), 4) != hipSuccess) return -1;
   if (clear) {
     u32 z = 0;
     if (hipMemcpyToSymbol(HIP_SYMBOL(g_error), &z, 4) != hipSuccess) return -1;
   }
-  *out = v;
+  if (batched_error_flag(&vb, clear)) return -1;
+  *out = v | vb;
   return 0;
 }

@@ -352,13 +352,17 @@ int jqh_validate(const uint64_t *bs, uint64_t n_bs, const uint32_t *bd, uint64_t
           if ((hk >> 8) < S && t >= num_ios) bad("hook beyond num_ios");
         }
         const uint32_t *g = h + 256;
-        for (uint64_t r = 0; r < R; ++r)
-          for (uint32_t t = 0; t < 256; ++t) {
+        for (uint32_t t = 0; t < 256; ++t) {
+          uint32_t bits = 0;
+          for (uint64_t r = 0; r < R; ++r) {
             uint32_t idx = g[512 * r + 2 * t], m = g[512 * r + 2 * t + 1];
             if (!m) continue;
+            bits += (uint32_t)__builtin_popcount(m);
             uint64_t w = idx & 0x7fffffffu;
             if (w >= state_size || (x && xoff + w >= state_size)) bad("global read index");
           }
+          if (bits > 32) bad("more than 32 global-read bits in one thread");
+        }
         const uint32_t *sec = g + 512 * R;
         for (uint32_t s = 0; s < S; ++s, sec += 5120)
           for (uint32_t k = 0; k < 4096; ++k)

@@ -79,6 +79,25 @@ def fill_input_words(states, state_size, data_pos, flag_pos, values):
         S[:n, pos >> 5] |= np.uint32(1) << np.uint32(pos & 31)
 
 
+def input_word_count(script):
+    """Words of the state that hold primary inputs (and the constant-zero
+    padding bit), src/flatten.rs:1163-1170."""
+    n = len(script.input_layout)
+    w = (n + 31) // 32
+    return w + 1 if n == w * 32 else w
+
+
+def input_words(script, cycles, seed=1, p=0.5, first_cycle=0):
+    """uint32[cycles+1, input_word_count]: the input area of snapshots
+    first_cycle .. first_cycle+cycles."""
+    nw = input_word_count(script)
+    data, fl = input_roles(script)
+    vals = lane_inputs(len(data), first_cycle + cycles, seed, p)[first_cycle:]
+    words = np.zeros((cycles + 1) * nw, np.uint32)
+    fill_input_words(words, nw, data, fl, vals)
+    return words.reshape(cycles + 1, nw)
+
+
 def build_states(script, cycles, seed=1, p=0.5, initial_xmask=None):
     """(cycles+1) x effective_state_size snapshot buffer for the plain
     drop-in: input bits per snapshot, X masks (reference template, or an
@@ -86,9 +105,8 @@ def build_states(script, cycles, seed=1, p=0.5, initial_xmask=None):
     ss = script.effective_state_size()
     rio = script.reg_io_state_size
     states = np.zeros((cycles + 1) * ss, np.uint32)
-    data, fl = input_roles(script)
-    vals = lane_inputs(len(data), cycles, seed, p)
-    fill_input_words(states, ss, data, fl, vals)
+    iw = input_words(script, cycles, seed, p)
+    states.reshape(-1, ss)[:, :iw.shape[1]] = iw
     if script.xprop_enabled:
         S = states.reshape(-1, ss)
         S[:, rio:2 * rio] = script.xmask_template()[None, :]
