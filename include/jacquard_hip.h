@@ -116,16 +116,21 @@ int jq_simulate_batched(usize num_groups, usize blocks_per_group,
 int jq_max_resident_blocks(int kind, int *out_blocks);
 
 /* ---- Learned cost model MLP (no counterpart in the reference) -----------
- * Fully connected FP32 network in -> h1 -> h2 -> 1 with ReLU, row-major
- * weights W1[h1][in], b1[h1], W2[h2][h1], b2[h2], W3[h2], b3[1] packed in
- * that order in `params`.  Forward: y[n] for x[n][in].  Train step: one
- * mean-squared-error gradient over the batch, written to `grads` (same
- * packing), then params -= lr * grads.  All on `stream`. */
+ * Fully connected FP32 network in -> h1 -> h2 -> 1 with ReLU (in <= 32,
+ * h1, h2 <= 64), row-major weights W1[h1][in], b1[h1], W2[h2][h1], b2[h2],
+ * W3[h2], b3[1] packed in that order in `params`
+ * (jq_mlp_num_params floats).  Forward: y[n] for x[n][in].  Train step: the
+ * mean-squared-error gradient of the batch (deterministic: per-workgroup
+ * partial sums in `workspace`, jq_mlp_workspace_floats floats, reduced in a
+ * fixed order) written to `grads`, then params -= lr * grads, and the batch
+ * loss to *loss_out (device pointer, may be NULL).  All on `stream`. */
+size_t jq_mlp_num_params(int in, int h1, int h2);
+size_t jq_mlp_workspace_floats(int in, int h1, int h2, int n);
 int jq_mlp_forward(const float *params, int in, int h1, int h2,
                    const float *x, int n, float *y, void *stream);
-int jq_mlp_train_step(float *params, float *grads, int in, int h1, int h2,
-                      const float *x, const float *target, int n, float lr,
-                      float *loss_out, void *stream);
+int jq_mlp_train_step(float *params, float *grads, float *workspace, int in,
+                      int h1, int h2, const float *x, const float *target,
+                      int n, float lr, float *loss_out, void *stream);
 
 /* Last error message of this thread ("" if none). */
 const char *jq_last_error(void);

@@ -101,6 +101,8 @@ HIP_SYMBOLS = [
     "jq_max_resident_blocks",
     "jq_mlp_forward",
     "jq_mlp_train_step",
+    "jq_mlp_num_params",
+    "jq_mlp_workspace_floats",
     "jq_last_error",
 ]
 
@@ -135,10 +137,12 @@ def hip():
         "jq_mlp_forward": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, c_vp, ctypes.c_int,
                                           c_vp, c_vp]),
-        "jq_mlp_train_step": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int,
+        "jq_mlp_train_step": (ctypes.c_int, [c_vp, c_vp, c_vp, ctypes.c_int,
                                              ctypes.c_int, ctypes.c_int,
                                              c_vp, c_vp, ctypes.c_int,
                                              ctypes.c_float, c_vp, c_vp]),
+        "jq_mlp_num_params": (ctypes.c_size_t, [ctypes.c_int] * 3),
+        "jq_mlp_workspace_floats": (ctypes.c_size_t, [ctypes.c_int] * 4),
         "jq_last_error": (c_cp, []),
         "jq_device_error_flag": (ctypes.c_int, [P(c_u32), ctypes.c_int]),
     }

@@ -27,7 +27,7 @@
 
 namespace jqd {
 
-__device__ u32 gb_barrier[kBarrierSlots * 2];
+__device__ u32 gb_barrier[kBarrierSlots * kSlotWords];
 __device__ u32 gb_error;
 
 namespace {
@@ -276,7 +276,7 @@ __device__ void run_block_lanes(const BArgs &A, const u32 *__restrict__ script,
 __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
   extern __shared__ u32 lds[];
   const u32 g = blockIdx.x / A.W, b = blockIdx.x % A.W;
-  u32 *bar = A.bar + 2 * g;
+  u32 *bar = A.bar + kSlotWords * g;
   u32 gen = read_gen(bar);
   const u32 ncyc = A.gcycles[g];
   const size_t snap = (size_t)A.rio * 32;
@@ -402,7 +402,7 @@ extern "C" int jq_simulate_batched(
   }
   u32 first = __atomic_fetch_add(&b_slot, (u32)num_groups, __ATOMIC_RELAXED) % kBarrierSlots;
   if (first + num_groups > (usize)kBarrierSlots) first = 0;
-  A.bar = bar_base + 2 * first;
+  A.bar = bar_base + kSlotWords * first;
   void *args[] = {&A};
   hipError_t e = hipLaunchCooperativeKernel(
       reinterpret_cast<const void *>(&batched_kernel),

@@ -31,18 +31,24 @@ int batched_error_flag(u32 *out, int clear);
 // device-scope acquire (these also write back / invalidate the per-XCD L2
 // state that other workgroups read).  The wait is bounded: after ~2^27
 // polls it raises g_error and returns so the grid always drains.
+// A slot is kSlotWords u32 (256 bytes): count at [0], generation at [32],
+// so concurrent barriers never share a cache line.
+constexpr int kSlotWords = 64;
+
 __device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen, u32 *err) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    u32 arrived = atomicAdd(&bar[0], 1u);
+    // the arrival is an agent-scope acq_rel RMW (releases this workgroup's
+    // stores, ordered after them by the barrier above); the waiters'
+    // acquire loads make every other workgroup's stores visible
+    u32 arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL,
+                                         __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == n - 1) {
-      atomicExch(&bar[0], 0u);
-      __threadfence();
-      atomicAdd(&bar[1], 1u);
+      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bar[32], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       u32 spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE,
+      while (__hip_atomic_load(&bar[32], __ATOMIC_ACQUIRE,
                                __HIP_MEMORY_SCOPE_AGENT) == gen) {
         __builtin_amdgcn_s_sleep(1);
         if (++spins > (1u << 27)) {
@@ -51,14 +57,13 @@ __device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen, u32 *err) {
         }
       }
     }
-    __threadfence();
     gen += 1;
   }
   __syncthreads();
 }
 
 __device__ __forceinline__ u32 read_gen(const u32 *bar) {
-  return __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(&bar[32], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One AND-with-invert word of the boomerang tree

@@ -26,7 +26,7 @@
 
 namespace jqd {
 
-__device__ u32 g_barrier[kBarrierSlots * 2];
+__device__ u32 g_barrier[kBarrierSlots * kSlotWords];
 __device__ u32 g_error;
 
 namespace {
@@ -442,7 +442,7 @@ int launch_scan(ScanArgs &A, hipStream_t stream) {
   u32 *bar_base = nullptr;
   if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(g_barrier)) != hipSuccess)
     return fail("hipGetSymbolAddress(g_barrier) failed");
-  A.bar = bar_base + 2 * (__atomic_fetch_add(&g_slot, 1u, __ATOMIC_RELAXED) % kBarrierSlots);
+  A.bar = bar_base + kSlotWords * (__atomic_fetch_add(&g_slot, 1u, __ATOMIC_RELAXED) % kBarrierSlots);
   void *args[] = {&A};
   hipError_t e = hipLaunchCooperativeKernel(
       reinterpret_cast<const void *>(&scan_kernel<TIMING, PROFILE>),

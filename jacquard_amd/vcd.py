@@ -1,0 +1,426 @@
+"""VCD stimulus reader and output writers with the reference's semantics.
+
+Reader: ``parse_input_vcd`` follows src/sim/vcd_io.rs:347-455 -- a snapshot
+is pushed at the first timestamp after one at which a clock edge happened,
+data inputs change immediately, clock-pin changes are applied one
+timestamp later, edge flags are cleared after every pushed snapshot, and
+vector values are applied character by character from the MSB side
+(``b11`` on a 4-bit vector sets bits 3 and 2 and leaves bits 1 and 0 as
+they were, src/sim/vcd_io.rs:279-330, 403-443).
+
+Writers: ``write_output_vcd`` (plain / X) and ``write_output_vcd_timed``
+follow src/sim/vcd_io.rs:561-946 and the vcd-ng/rust-vcd text layout
+(identifier codes '!'..'~' in base 94; beyond 94 wires this encoding is the
+rust-vcd convention and is not pinned by any reference fixture).
+"""
+import re
+
+import numpy as np
+
+U64_MAX = (1 << 64) - 1
+_PS_DIVISOR = {"s": 1, "ms": 10**3, "us": 10**6, "ns": 10**9, "ps": 10**12, "fs": 10**15}
+
+
+# ---------------------------------------------------------------- header
+class Var:
+    def __init__(self, vtype, size, code, ref, index):
+        self.vtype, self.size, self.code, self.ref, self.index = vtype, size, code, ref, index
+
+
+class Scope:
+    def __init__(self, kind, name, parent=None):
+        self.kind, self.name, self.parent = kind, name, parent
+        self.children = []  # Var or Scope
+
+    def path(self):
+        p, s = [], self
+        while s is not None and s.name is not None:
+            p.append(s.name)
+            s = s.parent
+        return "/".join(reversed(p))
+
+
+class Header:
+    def __init__(self):
+        self.timescale = None  # (ratio, unit)
+        self.root = Scope("root", None)
+
+
+def _tokens(text):
+    return text.split()
+
+
+def parse_header(text):
+    """Returns (Header, body_text)."""
+    m = re.search(r"\$enddefinitions\s+\$end", text)
+    if not m:
+        raise ValueError("VCD: no $enddefinitions")
+    head, body = text[:m.end()], text[m.end():]
+    toks = _tokens(head)
+    h = Header()
+    cur = h.root
+    i = 0
+    while i < len(toks):
+        t = toks[i]
+        if t == "$timescale":
+            j = toks.index("$end", i)
+            spec = "".join(toks[i + 1:j])
+            mm = re.match(r"(\d+)\s*([a-z]+)", spec)
+            h.timescale = (int(mm.group(1)), mm.group(2))
+            i = j + 1
+        elif t == "$scope":
+            s = Scope(toks[i + 1], toks[i + 2], cur)
+            cur.children.append(s)
+            cur = s
+            i = toks.index("$end", i) + 1
+        elif t == "$upscope":
+            cur = cur.parent
+            i = toks.index("$end", i) + 1
+        elif t == "$var":
+            j = toks.index("$end", i)
+            f = toks[i + 1:j]
+            vtype, size, code, ref = f[0], int(f[1]), f[2], f[3]
+            index = None
+            rest = "".join(f[4:])
+            if not rest and "[" in ref and ref.endswith("]"):
+                ref, rest = ref[:ref.index("[")], ref[ref.index("["):]
+            if rest:
+                mm = re.match(r"\[(-?\d+)(?::(-?\d+))?\]", rest)
+                if mm.group(2) is None:
+                    index = ("bit", int(mm.group(1)))
+                else:
+                    index = ("range", int(mm.group(1)), int(mm.group(2)))
+            cur.children.append(Var(vtype, size, code, ref, index))
+            i = j + 1
+        elif t.startswith("$"):
+            try:
+                i = toks.index("$end", i) + 1
+            except ValueError:
+                i += 1
+        else:
+            i += 1
+    return h, body
+
+
+def _scopes(s):
+    for c in s.children:
+        if isinstance(c, Scope):
+            yield c
+            yield from _scopes(c)
+
+
+def find_scope(header, path):
+    for s in _scopes(header.root):
+        if s.path() == path.strip("/"):
+            return s
+    raise ValueError(f"VCD scope {path!r} not found")
+
+
+def _range(a, b):
+    step = 1 if b >= a else -1
+    return list(range(a, b + step, step))
+
+
+def match_vcd_inputs(scope, input_names):
+    """{(code, pos): port bit name} for the scope's vars that name input
+    ports (src/sim/vcd_io.rs:279-344)."""
+    names = set(input_names)
+    out = {}
+    for v in scope.children:
+        if not isinstance(v, Var):
+            continue
+
+        def one(bitname, pos):
+            if bitname in names:
+                out[(v.code, pos)] = bitname
+
+        if v.index is None:
+            if v.size == 1:
+                one(v.ref, 0)
+            else:
+                for pos, i in enumerate(reversed(range(v.size))):
+                    one(f"{v.ref}[{i}]", pos)
+        elif v.index[0] == "bit":
+            one(f"{v.ref}[{v.index[1]}]", 0)
+        else:
+            for pos, i in enumerate(_range(v.index[1], v.index[2])):
+                one(f"{v.ref}[{i}]", pos)
+    return out
+
+
+def auto_scope(header, input_names):
+    """The first scope (depth first) whose variables cover every input."""
+    best = None
+    for s in _scopes(header.root):
+        m = match_vcd_inputs(s, input_names)
+        covered = set(m.values())
+        if covered >= set(input_names):
+            return s
+        if best is None or len(covered) > best[0]:
+            best = (len(covered), s)
+    if best is None:
+        raise ValueError("VCD: no scope")
+    return best[1]
+
+
+# ---------------------------------------------------------------- reader
+def parse_input_vcd(text, script, scope=None, max_clock_edges=None):
+    """(input_states uint32[(C+1)*rio], offsets_timestamps [(offset, ts)])."""
+    netlist = script.netlist
+    ins, _ = netlist.ports()
+    in_names = [n for (n, _, _) in ins]
+    header, body = parse_header(text)
+    sc = find_scope(header, scope) if scope else auto_scope(header, in_names)
+    vcd2inp = match_vcd_inputs(sc, in_names)
+    name2port = {n: (pin, flag) for (n, pin, flag) in ins}
+    rio = script.reg_io_state_size
+    state = np.zeros(rio, np.uint32)
+    # (code, pos) -> (state bit, flag bit or None)
+    code_map = {}
+    for (code, pos), name in vcd2inp.items():
+        pin, flag = name2port[name]
+        bit = script.input_map[pin]
+        fbit = script.input_map[flag] if flag else None
+        code_map.setdefault(code, {})[pos] = (bit, fbit)
+    flag_bits = [script.input_map[f] for (_, _, f) in ins if f]
+
+    def getb(p):
+        return int(state[p >> 5] >> np.uint32(p & 31)) & 1
+
+    def setb(p, v):
+        if v:
+            state[p >> 5] |= np.uint32(1) << np.uint32(p & 31)
+        else:
+            state[p >> 5] &= ~(np.uint32(1) << np.uint32(p & 31))
+
+    snaps, offs = [], []
+    vcd_time_last_active = U64_MAX
+    vcd_time = 0
+    last_active = True
+    delayed = []
+    toks = body.split()
+    i = 0
+    while i < len(toks):
+        tok = toks[i]
+        i += 1
+        c0 = tok[0]
+        if c0 == "#":
+            t = int(tok[1:])
+            if t == vcd_time:
+                continue
+            if last_active:
+                snaps.append(state.copy())
+                offs.append((len(snaps) * rio, vcd_time_last_active))
+                for fb in flag_bits:
+                    setb(fb, 0)
+                if max_clock_edges is not None and len(offs) >= max_clock_edges:
+                    break
+                vcd_time_last_active = vcd_time
+            vcd_time = t
+            last_active = False
+            for p in dict.fromkeys(delayed):
+                state[p >> 5] ^= np.uint32(1) << np.uint32(p & 31)
+            delayed = []
+            continue
+        if c0 == "$":
+            if tok == "$comment":
+                while i < len(toks) and toks[i] != "$end":
+                    i += 1
+                i += 1
+            continue  # $dumpvars / $dumpall / $end markers
+        if c0 in "bBrR":
+            # vector or real value: the identifier is the next token
+            bits, code = tok[1:], toks[i]
+            i += 1
+            if c0 in "rR":
+                continue
+        else:
+            bits, code = tok[0], tok[1:]
+        m = code_map.get(code)
+        if not m:
+            continue
+        for pos, b in enumerate(bits):
+            e = m.get(pos)
+            if e is None:
+                continue
+            bit, fbit = e
+            old = getb(bit)
+            new = 1 if b == "1" else 0
+            if old == new:
+                continue
+            if fbit is not None:
+                if old == 0:
+                    last_active = True
+                    setb(fbit, 1)
+                delayed.append(bit)
+            else:
+                state[bit >> 5] ^= np.uint32(1) << np.uint32(bit & 31)
+    snaps.append(state.copy())
+    return np.concatenate(snaps), offs, header
+
+
+# ---------------------------------------------------------------- writers
+def id_code(i):
+    """rust-vcd IdCode display: base-94 over '!'..'~'."""
+    s = []
+    while True:
+        r = i % 94
+        s.append(chr(33 + r))
+        if i < 94:
+            break
+        i = i // 94 - 1
+    return "".join(s)
+
+
+def output_mapping(script):
+    """[(literal, state bit or None, name)] in output port order, skipping
+    outputs that pass an input port through (src/sim/vcd_io.rs:580-613)."""
+    A = script.netlist.arrays()
+    _, outs = script.netlist.ports()
+    res = []
+    for name, lit in outs:
+        if lit > 1 and A["drv"][lit >> 1] == 2:  # input port pass-through
+            continue
+        if lit <= 1:
+            res.append((lit, None, name))
+        else:
+            res.append((lit, script.output_map[lit], name))
+    return res
+
+
+def _header_text(timescale, scope, mapping):
+    lines = []
+    if timescale:
+        lines.append(f"$timescale {timescale[0]} {timescale[1]} $end")
+    parts = scope.split("/")
+    for p in parts:
+        lines.append(f"$scope module {p} $end")
+    for i, (_, _, name) in enumerate(mapping):
+        lines.append(f"$var wire 1 {id_code(i)} {name} $end")
+    for _ in parts:
+        lines.append("$upscope $end")
+    lines.append("$enddefinitions $end")
+    lines.append("$dumpvars")
+    return lines
+
+
+def _values(states, rio, offset, mapping, xmask):
+    vals = []
+    for lit, pos, _ in mapping:
+        if pos is None:
+            vals.append(lit)
+            continue
+        w = states[offset + (pos >> 5)]
+        v = int(w >> np.uint32(pos & 31)) & 1
+        if xmask is not None and (int(xmask[offset + (pos >> 5)] >> np.uint32(pos & 31)) & 1):
+            vals.append(3)
+        else:
+            vals.append(v)
+    return vals
+
+
+_SYM = {0: "0", 1: "1", 3: "x"}
+
+
+def write_output_vcd(script, offsets_timestamps, states, timescale=(1, "ps"),
+                     scope="gem_top_module", xmask_states=None):
+    """Text of the output VCD (write_output_vcd[_xprop],
+    src/sim/vcd_io.rs:729-820).  states / xmask_states: value-only buffers
+    laid out like parse_input_vcd's input_states."""
+    mapping = output_mapping(script)
+    lines = _header_text(timescale, scope, mapping)
+    last = [2] * len(mapping)
+    rio = script.reg_io_state_size
+    for offset, ts in offsets_timestamps:
+        if ts == U64_MAX:
+            continue
+        lines.append(f"#{ts}")
+        vals = _values(states, rio, offset, mapping, xmask_states)
+        for i, v in enumerate(vals):
+            if v == last[i]:
+                continue
+            last[i] = v
+            lines.append(f"{_SYM[v]}{id_code(i)}")
+    return "\n".join(lines) + "\n"
+
+
+def write_output_vcd_timed(script, offsets_timestamps, states, arrival_states,
+                           timescale=(1, "ps"), scope="gem_top_module",
+                           xmask_states=None):
+    """Timed VCD (src/sim/vcd_io.rs:831-946): every transition is shifted by
+    its word's arrival (ps, rounded to VCD units), transitions are stably
+    sorted by time."""
+    mapping = output_mapping(script)
+    lines = _header_text(timescale, scope, mapping)
+    if timescale:
+        ps_per_unit = timescale[0] * (_PS_DIVISOR["ps"] / _PS_DIVISOR[timescale[1]])
+    else:
+        ps_per_unit = 1.0
+    last = [2] * len(mapping)
+    trans = []
+    for offset, ts in offsets_timestamps:
+        if ts == U64_MAX:
+            continue
+        vals = _values(states, script.reg_io_state_size, offset, mapping, xmask_states)
+        for i, v in enumerate(vals):
+            if v == last[i]:
+                continue
+            last[i] = v
+            pos = mapping[i][1]
+            arr = 0 if pos is None else int(arrival_states[offset + (pos >> 5)]) & 0xFFFF
+            units = int(_round_half_away(arr / ps_per_unit)) if ps_per_unit > 0 else 0
+            trans.append((ts + units, i, v))
+    trans.sort(key=lambda x: x[0])
+    cur = None
+    for ts, i, v in trans:
+        if ts != cur:
+            lines.append(f"#{ts}")
+            cur = ts
+        lines.append(f"{_SYM[v]}{id_code(i)}")
+    return "\n".join(lines) + "\n"
+
+
+def _round_half_away(x):
+    """f64::round semantics (half away from zero)."""
+    import math
+    return math.floor(x + 0.5) if x >= 0 else -math.floor(-x + 0.5)
+
+
+# ---------------------------------------------------------------- buffers
+def expand_states_for_xprop(value_states, script, initial_xmask=None):
+    """[values | X mask] per snapshot, template X (src/sim/vcd_io.rs:465-484),
+    optionally an explicit X mask for snapshot 0."""
+    rio = script.reg_io_state_size
+    n = value_states.size // rio
+    out = np.zeros((n, 2 * rio), np.uint32)
+    out[:, :rio] = value_states.reshape(n, rio)
+    out[:, rio:] = script.xmask_template()[None, :]
+    if initial_xmask is not None:
+        out[0, rio:] = initial_xmask
+    return out.reshape(-1)
+
+
+def expand_states_for_arrivals(states, script):
+    """Append a zeroed arrival section per snapshot
+    (src/sim/vcd_io.rs:511-530)."""
+    old = script.reg_io_state_size * (2 if script.xprop_enabled else 1)
+    rio = script.reg_io_state_size
+    n = states.size // old
+    out = np.zeros((n, old + rio), np.uint32)
+    out[:, :old] = states.reshape(n, old)
+    return out.reshape(-1)
+
+
+def split_sections(gpu_states, script):
+    """(values, xmask or None, arrivals or None), each laid out like the
+    value-only buffer (src/sim/vcd_io.rs:491-550)."""
+    ss = script.effective_state_size()
+    rio = script.reg_io_state_size
+    S = gpu_states.reshape(-1, ss)
+    vals = S[:, :rio].reshape(-1).copy()
+    xm = S[:, rio:2 * rio].reshape(-1).copy() if script.xprop_enabled else None
+    arr = None
+    if script.timing_arrivals_enabled:
+        a0 = script.arrival_state_offset
+        arr = S[:, a0:a0 + rio].reshape(-1).copy()
+    return vals, xm, arr

@@ -734,7 +734,10 @@ int jqo_aig_sim(const jqo_aig *g, size_t num_cycles, size_t num_inputs,
     memx[i] = xprop ? 0xFFFFFFFFu : 0;
   for (u32 d = 0; d < g->num_dffs; ++d)
     reg[d].x = (uint8_t)(xprop && dff_init_x && dff_init_x[d]);
-  for (u32 s = 0; s < g->num_srams * 32; ++s) rd[s].x = (uint8_t)(xprop != 0);
+  /* SRAM read-data registers start known, as the reference's X template
+   * clears every input_map bit (src/sim/vcd_io.rs:470-474); the memory
+   * array itself starts all X (src/bin/jacquard.rs:1293-1295). */
+  for (u32 s = 0; s < g->num_srams * 32; ++s) rd[s].x = 0;
 
   for (size_t c = 0; c < num_cycles; ++c) {
     /* sources */

@@ -1,0 +1,120 @@
+"""C ABI exports, VCD writer details and the timing report (CPU only)."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+
+from jacquard_amd import _native as N
+from jacquard_amd import events, vcd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    with open(os.path.join(ROOT, "include", "jacquard_hip.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", text)
+
+
+def test_hip_library_exports_every_declared_symbol():
+    names = [n for n in _declared_functions() if n not in ("if", "sizeof")]
+    assert "simulate_v1_noninteractive_simple_scan_hip" in names
+    assert "simulate_v1_noninteractive_timed_hip" in names
+    lib = ctypes.CDLL(os.path.join(ROOT, "jacquard_amd", "lib", "libjacquard_hip.so"))
+    missing = []
+    for n in names:
+        try:
+            getattr(lib, n)
+        except AttributeError:
+            missing.append(n)
+    assert not missing, missing
+    assert N.missing_hip_symbols() == []
+
+
+def test_id_codes():
+    assert [vcd.id_code(i) for i in (0, 1, 93)] == ["!", '"', "~"]
+    assert vcd.id_code(94) == "!!"
+    assert vcd.id_code(95) == '"!'
+
+
+class _FakeScript:
+    def __init__(self, rio):
+        self.reg_io_state_size = rio
+
+
+def _timed(mapping, offs, states, arr, ts):
+    import types
+    s = _FakeScript(2)
+    orig = vcd.output_mapping
+    try:
+        vcd.output_mapping = lambda script: mapping
+        return vcd.write_output_vcd_timed(s, offs, np.asarray(states, np.uint32),
+                                          np.asarray(arr, np.uint32), ts, "test")
+    finally:
+        vcd.output_mapping = orig
+
+
+def test_timed_vcd_offsets():
+    """src/sim/vcd_io.rs:1335-1400: A at 350 ps, B at 885 ps after 10000."""
+    out = _timed([(2, 0, "A"), (4, 32, "B")], [(0, 0), (2, 10000)],
+                 [0, 0, 1, 1], [0, 0, 350, 885], (1, "ps"))
+    assert "#10350" in out and "#10885" in out and "#0" in out
+    assert out.index("#10350") < out.index("#10885")
+
+
+def test_timed_vcd_ns_rounding():
+    """500 ps at a 1 ns timescale rounds to one unit: t = 11."""
+    out = _timed([(2, 0, "A")], [(0, 0), (1, 10)], [0, 1], [0, 500], (1, "ns"))
+    assert "#11\n" in out
+
+
+def test_timed_vcd_zero_arrival():
+    out = _timed([(2, 0, "A")], [(0, 0), (1, 5000)], [0, 1], [0, 0], (1, "ps"))
+    assert "#5000\n1!" in out
+
+
+def _ev(cycle, kind, word, slack, arrival, lim):
+    e = np.zeros(1, events.np.dtype([("event_type", "<u4"), ("message_id", "<u4"),
+                                     ("cycle", "<u4"), ("reserved", "<u4"),
+                                     ("data", "<u4", (4,))]))
+    e["event_type"] = kind
+    e["message_id"] = word
+    e["cycle"] = cycle
+    e["data"] = [word, np.int32(slack).view(np.uint32), arrival, lim]
+    return e
+
+
+def test_timing_report_matches_sample_shape():
+    """The three violations of tests/timing_ir/sample_reports/
+    two_violations.json rebuilt from raw device events (given in a scrambled
+    order, as atomics would)."""
+    ev = np.concatenate([_ev(91, 5, 12, -40, 10, 50), _ev(87, 4, 5, -150, 950, 200),
+                         _ev(42, 4, 5, -100, 900, 200)])
+    sites = {5: "top/cpu/regs[7][bit 22] [word=5]", 12: "top/cpu/state[bit 3] [word=12]"}
+    rep = events.build_report(ev, 1000, 1000, design="tiny_cpu.gv",
+                              vector_source="boot.vcd", timing_source="tiny_cpu.jtir",
+                              describe=sites.get)
+    assert rep["schema_version"] == "1.1.0"
+    assert rep["stats"] == {"setup_violations": 2, "hold_violations": 1,
+                            "events_dropped": 0, "violations_truncated": 0}
+    assert [v["cycle"] for v in rep["violations"]] == [42, 87, 91]
+    assert [w["word_id"] for w in rep["per_word"]] == [5, 12]
+    assert rep["per_word"][0]["worst_setup_slack_ps"] == -150
+    assert rep["per_word"][0]["worst_hold_slack_ps"] is None
+    assert rep["per_word"][0]["worst_arrival_ps"] == 950
+    assert [v["slack_ps"] for v in rep["worst_slack"]["setup"]] == [-150, -100]
+    assert rep["worst_slack"]["hold"][0]["slack_ps"] == -40
+    json.loads(events.report_json(rep))
+    text = events.format_summary(rep)
+    assert "Setup: 2" in text and "Hold:  1" in text
+
+
+def test_report_truncation():
+    ev = np.concatenate([_ev(c, 5, 1, -1, 0, 1) for c in range(5)])
+    rep = events.build_report(ev, 5, 100, max_violations=3)
+    assert len(rep["violations"]) == 3
+    assert rep["stats"]["violations_truncated"] == 2
+    assert rep["stats"]["hold_violations"] == 5

@@ -1,0 +1,99 @@
+"""Compiler + oracle against the direct AIG simulation (closed form).
+
+For any valid script the interpreted values must equal a direct cycle
+simulation of the AIG, which never looks at the script; this pins the
+compiler (levelization, level split, partitioning, mapping, flatten) and the
+oracle together.  Also checks the literal 2-state restatement against the
+kernel-semantics oracle, the script validator, and layout invariants of the
+reference format.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import CONFIG1, design, direct_check
+from jacquard_amd import stimulus
+from jacquard_amd.script import SRAM_WORDS
+
+
+@pytest.mark.parametrize("blocks,split", [(1, 0), (4, 0), (8, 6), (32, 3)])
+def test_config1_vs_direct(blocks, split):
+    nl, comp = design("c1", level_split=split, **CONFIG1)
+    s = comp.flatten(blocks)
+    s.validate()
+    cycles = 120
+    st = stimulus.build_states(s, cycles, seed=3, p=0.5)
+    a, b = st.copy(), st.copy()
+    oracle.run(s, a, cycles)
+    oracle.run_v1(s, b, cycles)
+    assert np.array_equal(a, b)
+    direct_check(s, a, cycles, 3, 0.5)
+
+
+@pytest.mark.parametrize("params", [
+    dict(seed=21, gates=6000, depth=30, dffs=600, inputs=48, outputs=24,
+         geometric_p=0.3, srams=2, level_split=5),
+    dict(seed=22, gates=4000, depth=12, dffs=300, inputs=8, outputs=8,
+         geometric_p=0.0, srams=1, level_split=0),
+])
+def test_srams_xprop_vs_direct(params):
+    nl, comp = design("mix", **params)
+    for xprop in (False, True):
+        s = comp.flatten(8, xprop=xprop)
+        s.validate()
+        cycles = 60
+        m, pick = (None, None)
+        if xprop:
+            m, pick = stimulus.dff_initial_xmask(s, 0.1, seed=4)
+        st = stimulus.build_states(s, cycles, seed=5, p=0.3, initial_xmask=m)
+        sram = np.zeros(s.sram_storage_size, np.uint32)
+        sram_x = np.full(s.sram_storage_size, 0xFFFFFFFF, np.uint32) if xprop else None
+        oracle.run(s, st, cycles, sram=sram, sram_x=sram_x)
+        direct_check(s, st, cycles, 5, 0.3, xprop=xprop,
+                     dff_init_x=None if pick is None else pick.astype(np.uint8))
+
+
+def test_layout_invariants():
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    s = comp.flatten(8)
+    bs, bd = s.blocks_start, s.blocks_data
+    assert bs.size == s.num_blocks * s.num_major_stages + 1
+    assert bd.size == int(bs[-1]) + 2048          # padding, flatten.rs:1341
+    nin = len(s.input_layout)
+    start = (nin + 31) // 32 + (1 if nin % 32 == 0 else 0)
+    # writeouts begin after the input words (flatten.rs:1163-1170)
+    ios = sorted((int(bd[o + 3]), int(bd[o + 2])) for o in s.parts[:, 3])
+    assert ios[0][0] == start
+    for (a, n), (b, _) in zip(ios, ios[1:]):
+        assert a + n == b                          # contiguous, block order
+    assert ios[-1][0] + ios[-1][1] == s.reg_io_state_size
+    for o in s.parts[:, 3]:
+        S = int(bd[o])
+        R = int(bd[o + 6])
+        assert int(s.parts[list(s.parts[:, 3]).index(o), 4]) == 256 + 512 * R + 5120 * S + 10240
+
+
+def test_validator_rejects_corruption():
+    nl, comp = design("c1", level_split=0, **CONFIG1)
+    s = comp.flatten(2)
+    s.validate()
+    o = int(s.parts[0, 3])
+    saved = int(s.blocks_data[o + 3])
+    s.blocks_data[o + 3] = s.reg_io_state_size  # writeout past the state
+    with pytest.raises(ValueError):
+        s.validate()
+    s.blocks_data[o + 3] = saved
+    g = o + 256
+    saved = int(s.blocks_data[g + 1])
+    s.blocks_data[g + 1] = 0xFFFFFFFF
+    s.blocks_data[g] = s.reg_io_state_size + 5  # global read past the state
+    with pytest.raises(ValueError):
+        s.validate()
+
+
+def test_sram_storage_layout():
+    nl, comp = design("mix", seed=21, gates=6000, depth=30, dffs=600, inputs=48,
+                      outputs=24, geometric_p=0.3, srams=2, level_split=5)
+    s = comp.flatten(4)
+    assert s.sram_storage_size == 2 * SRAM_WORDS
+    assert sorted(s.sram_offsets.tolist()) == [0, SRAM_WORDS]
