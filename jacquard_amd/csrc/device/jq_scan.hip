@@ -133,19 +133,39 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
     hook = (t & 1) ? (hook >> 16) : (hook & 0xffffu);
     pi += 256;
 
-    // ---- global read (pext-gather), kernel_v1_impl.cuh:87-151
+    // ---- global read (pext-gather), kernel_v1_impl.cuh:87-151.  Every
+    // load whose address is known from the header is issued here, before
+    // the first wait: the read entries (up to kPreRounds rounds), then the
+    // stage-0 permutation, both writeout permutation sections and the
+    // commit's old values, so one partition costs two dependent memory
+    // round trips before its first boomerang stage instead of six.
+    constexpr u32 kPreRounds = 16;
+    const uint2 *gr = reinterpret_cast<const uint2 *>(script + pi);
+    uint2 e[kPreRounds];
+#pragma unroll
+    for (u32 k = 0; k < kPreRounds; ++k)
+      e[k] = (k < rounds) ? gr[256 * k + t] : make_uint2(0, 0);
+    const size_t p_st = pi + (size_t)512 * rounds;
+    const size_t p_sd = p_st + (size_t)5120 * num_stages;
+    const size_t p_ck = p_sd + 5120;
+    uint4 q0 = ld4(script + p_st + 4 * t), q1 = ld4(script + p_st + 1024 + 4 * t);
+    uint4 q2 = ld4(script + p_st + 2048 + 4 * t), q3 = ld4(script + p_st + 3072 + 4 * t);
+    const uint4 sd0 = ld4(script + p_sd + 4 * t), sd1 = ld4(script + p_sd + 1024 + 4 * t);
+    const uint4 sd2 = ld4(script + p_sd + 2048 + 4 * t), sd3 = ld4(script + p_sd + 3072 + 4 * t);
+    const uint4 sdf = ld4(script + p_sd + 4096 + 4 * t);
+    const uint4 ck0 = ld4(script + p_ck + 4 * t), ck1 = ld4(script + p_ck + 1024 + 4 * t);
+    const uint4 ck2 = ld4(script + p_ck + 2048 + 4 * t), ck3 = ld4(script + p_ck + 3072 + 4 * t);
+    const uint4 ckf = ld4(script + p_ck + 4096 + 4 * t);
+    u32 old_v = 0, old_x = 0;
+    if ((u32)t < num_ios) {
+      old_v = in[io_offset + (u32)t];
+      if (is_x) old_x = in[xoff + io_offset + (u32)t];
+    }
     u32 acc_v = 0, acc_x = 0;
-    for (u32 r = 0; r < rounds; r += 4) {
-      uint2 e[4];
-      u32 v[4], x[4];
+    {
+      u32 v[kPreRounds], x[kPreRounds];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        e[k] = (r + k < rounds)
-                   ? reinterpret_cast<const uint2 *>(script + pi + 512 * (r + k))[t]
-                   : make_uint2(0, 0);
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (u32 k = 0; k < kPreRounds; ++k) {
         v[k] = 0;
         x[k] = 0;
         if (e[k].y) {
@@ -156,7 +176,7 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
         }
       }
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (u32 k = 0; k < kPreRounds; ++k) {
         u32 m = e[k].y;
         while (m) {
           u32 low = m & (0u - m);
@@ -166,15 +186,27 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
         }
       }
     }
-    pi += (size_t)512 * rounds;
+    for (u32 r = kPreRounds; r < rounds; ++r) {  // rare: very wide reads
+      uint2 er = gr[256 * r + t];
+      if (!er.y) continue;
+      const u32 *base = (er.x >> 31) ? out : in;
+      u32 w = er.x & 0x7fffffffu;
+      u32 v = base[w], x = is_x ? base[xoff + w] : 0u;
+      u32 m = er.y;
+      while (m) {
+        u32 low = m & (0u - m);
+        acc_v = (acc_v << 1) | ((v & low) ? 1u : 0u);
+        acc_x = (acc_x << 1) | ((x & low) ? 1u : 0u);
+        m ^= low;
+      }
+    }
+    pi = p_st;
     st_v[t] = acc_v;
     st_x[t] = acc_x;
     if (TIMING) wo_arr[t] = 0;  // pinned rule: reset per partition
     __syncthreads();
 
     // ---- boomerang stages
-    uint4 q0 = ld4(script + pi + 4 * t), q1 = ld4(script + pi + 1024 + 4 * t);
-    uint4 q2 = ld4(script + pi + 2048 + 4 * t), q3 = ld4(script + pi + 3072 + 4 * t);
     for (u32 bs = 0; bs < num_stages; ++bs) {
       u32 hv = gather_bits(st_v, q0, q1, q2, q3);
       u32 hx = is_x ? gather_bits(st_x, q0, q1, q2, q3) : 0u;
@@ -300,14 +332,9 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
     // ---- SRAM + duplicate permutation, kernel_v1_impl.cuh:365-443
     const u32 nperm = num_srams * 4 + num_dups;
     u32 pv = 0, px = 0;
-    {
-      uint4 p0 = ld4(script + pi + 4 * t), p1 = ld4(script + pi + 1024 + 4 * t);
-      uint4 p2 = ld4(script + pi + 2048 + 4 * t), p3 = ld4(script + pi + 3072 + 4 * t);
-      uint4 f = ld4(script + pi + 4096 + 4 * t);
-      if ((u32)t < nperm) {
-        pv = (gather_bits(wo_v, p0, p1, p2, p3) & ~f.y) ^ f.x;
-        if (is_x) px = gather_bits(wo_x, p0, p1, p2, p3) & ~f.y;
-      }
+    if ((u32)t < nperm) {
+      pv = (gather_bits(wo_v, sd0, sd1, sd2, sd3) & ~sdf.y) ^ sdf.x;
+      if (is_x) px = gather_bits(wo_x, sd0, sd1, sd2, sd3) & ~sdf.y;
     }
     pi += 5120;
     const u32 en = __shfl(pv, (lane + 1) & 63);
@@ -344,18 +371,15 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
 
     // ---- clock-enable permutation + commit, kernel_v1_impl.cuh:446-558
     {
-      uint4 p0 = ld4(script + pi + 4 * t), p1 = ld4(script + pi + 1024 + 4 * t);
-      uint4 p2 = ld4(script + pi + 2048 + 4 * t), p3 = ld4(script + pi + 3072 + 4 * t);
-      uint4 f = ld4(script + pi + 4096 + 4 * t);
+      const uint4 f = ckf;
       if ((u32)t < num_ios) {
-        u32 ck = (gather_bits(wo_v, p0, p1, p2, p3) & ~f.y) ^ f.x;
+        u32 ck = (gather_bits(wo_v, ck0, ck1, ck2, ck3) & ~f.y) ^ f.x;
         u32 data = wo_v[t] ^ f.z;
         u32 w = io_offset + (u32)t;
-        out[w] = (in[w] & ~ck) | (data & ck);
+        out[w] = (old_v & ~ck) | (data & ck);
         if (is_x) {
-          u32 ckx = gather_bits(wo_x, p0, p1, p2, p3) & ~f.y;
-          u32 ox = in[xoff + w];
-          out[xoff + w] = (ox & ~ck & ~ckx) | (wo_x[t] & ck & ~ckx) | ckx;
+          u32 ckx = gather_bits(wo_x, ck0, ck1, ck2, ck3) & ~f.y;
+          out[xoff + w] = (old_x & ~ck & ~ckx) | (wo_x[t] & ck & ~ckx) | ckx;
         }
         if (TIMING) {
           u16 arr = wo_arr[t];
