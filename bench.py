@@ -53,14 +53,32 @@ def parse():
     ap.add_argument("--split", type=int, default=4, help="level-split period")
     ap.add_argument("--blocks", type=int, default=0, help="plain blocks (0 = auto)")
     ap.add_argument("--lanes", type=int, default=256)
-    ap.add_argument("--batched-scale", type=float, default=1.0,
-                    help="scale of the batched per-lane cycle counts")
+    ap.add_argument("--batched-scale", type=float, default=0.02,
+                    help="scale of the batched per-lane cycle counts "
+                         "(1.0 = U[10K, 200K])")
     ap.add_argument("--batched-blocks", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--verify-cycles", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run an elementwise copy of MIB MiB three times "
+                         "(the known-size kernel that scales the HBM counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """Three plain copies of `mib` MiB: each reads and writes exactly
+    mib * 2^20 bytes, which calibrates FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.neg(src, out=dst)  # an elementwise kernel, not a DMA blit
+    torch.cuda.synchronize()
+    del src, dst
+    log(f"copy calibration: 3 elementwise copies of {n * 4} bytes")
 
 
 def build_design(a):
@@ -206,12 +224,19 @@ def cpu_baseline(a, script):
     return dict(value=n / spent, cycles=n, seconds=spent, state_words=ss)
 
 
-def traffic_from_profile(cycles_per_launch):
-    path = os.path.join(ROOT, "profiles", "scan_kernel_config2_traffic.json")
-    if not os.path.exists(path):
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "scan_kernel_1M_gates_hbm_traffic.json")
+
+
+def traffic_from_profile(cycles_per_launch, script_bytes):
+    """HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE
+    profile of the same workload (already scaled by the copy-kernel
+    calibration), or None when no profile of this script is present."""
+    if not os.path.exists(TRAFFIC_PROFILE):
         return None
-    with open(path) as f:
+    with open(TRAFFIC_PROFILE) as f:
         p = json.load(f)
+    if p.get("script_bytes") != script_bytes:
+        return None
     per_cycle = p["hbm_bytes_per_launch"] / p["cycles_per_launch"]
     return per_cycle * cycles_per_launch
 
@@ -226,12 +251,16 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
     from jacquard_amd import _native as N
     import ctypes
     nl, comp = build_design(a)
     res = ctypes.c_int(0)
     N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
-    blocks = a.blocks or min(512, res.value)
+    # one block per partition of the widest major stage: more blocks only
+    # add grid-barrier arrivals, fewer serialise partitions
+    blocks = a.blocks or min(max(comp.parts_per_stage()), res.value)
     script = comp.flatten(blocks)
     log(f"plain script: {script.num_blocks} blocks, {script.num_major_stages} major stages, "
         f"{script.num_parts} partitions, {script.blocks_data.nbytes / 1e6:.1f} MB")
@@ -240,7 +269,7 @@ def main():
     avg_launch_s = float(np.mean(p["launch_ms"])) / 1e3
     bpc = script.algorithmic_bytes_per_cycle()
     achieved = bpc * a.cycles_per_step / avg_launch_s / 1e9
-    traffic = traffic_from_profile(a.cycles_per_step)
+    traffic = traffic_from_profile(a.cycles_per_step, int(script.blocks_data.nbytes))
     log(f"plain: {value:,.0f} lane-cycles/s, launch {avg_launch_s * 1e3:.1f} ms")
     forms = {"plain": {"lane_cycles_per_s": value, "verified_cycles": p.get("verified_cycles")}}
     if not a.no_batched:
