@@ -59,7 +59,7 @@ def build_host():
     srcs = [os.path.join(HERE, "csrc", "host", s) for s in HOST_SOURCES
             if os.path.exists(os.path.join(HERE, "csrc", "host", s))]
     out = os.path.join(LIB, "libjacquard_host.so")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall",
           "-Wno-unused-function", "-o", out] + srcs)
     return out
 

@@ -885,7 +885,11 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
     const MajorStage &ms = c.stages[m];
     std::vector<std::vector<uint32_t>> scripts(ms.parts.size());
     std::vector<std::vector<std::pair<uint64_t, std::vector<uint32_t>>>> patches(ms.parts.size());
-    for (size_t i = 0; i < ms.parts.size(); ++i) {
+    // partitions are independent here (shared maps are only read), so their
+    // sections are built in parallel; the first error is rethrown after
+    std::string par_err;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t i = 0; i < ms.parts.size(); ++i) try {
       const Partition &pt = ms.parts[i];
       const FlatPart &fp = fparts[m][i];
       std::vector<uint32_t> &sc = scripts[i];
@@ -1038,7 +1042,11 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
       };
       emit(fp.sd, false);
       emit(fp.ck, true);
+    } catch (const std::exception &e) {
+#pragma omp critical(jq_flatten_err)
+      if (par_err.empty()) par_err = e.what();
     }
+    if (!par_err.empty()) throw std::runtime_error(par_err);
     for (uint32_t blk = 0; blk < nb; ++blk) {
       s.blocks_start.push_back(s.blocks_data.size());
       const auto &lst = blocks_parts[m][blk];

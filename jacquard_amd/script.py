@@ -219,12 +219,14 @@ class FlattenedScriptV1:
     def algorithmic_bytes_per_cycle(self):
         """Algorithmic HBM bytes of one lane-cycle, from the script alone:
         script words + gathered state words + writeout read/write + SRAM
-        accesses (+X lane, +timing words)."""
+        accesses (+X lane, +arrival write and constraint read per writeout
+        word when timing arrivals are on)."""
         bd = self.blocks_data
         total_script = int(self.blocks_start[-1])
         g = 0
         io = 0
         sr = 0
+        tm = 0
         for off, length in self.parts[:, 3:5]:
             h = bd[int(off):int(off) + 256]
             R = int(h[6])
@@ -233,7 +235,9 @@ class FlattenedScriptV1:
             g += x * int(np.count_nonzero(gr[:, :, 1]))
             io += x * 2 * int(h[2])
             sr += x * 3 * int(h[4])
-        return 4 * (total_script + g + io + sr)
+            if self.timing_arrivals_enabled:
+                tm += 2 * int(h[2])
+        return 4 * (total_script + g + io + sr + tm)
 
 
 def compile_design(netlist, num_blocks, level_split=0, xprop=False, **kw):

@@ -101,3 +101,31 @@ def test_timing_events():
     a = sim.canonical_events(out["events"])
     b = sim.canonical_events(ev_ref)
     assert np.array_equal(a, b)
+
+
+def test_xprop_timing_srams_together():
+    """X lane, arrivals, setup/hold and SRAMs in one run (the 4M-gate
+    workload's feature mix at test size)."""
+    nl, comp = design("xts", level_split=6, seed=9, gates=40000, depth=48,
+                      dffs=3000, inputs=64, outputs=32, geometric_p=0.3, srams=2,
+                      delay=(20, 200))
+    s = comp.flatten(24, xprop=True)
+    s.inject_timing()
+    s.enable_timing_arrivals()
+    cons = s.build_timing_constraint_buffer(1500, 50, 20)
+    cycles = 60
+    m, _ = stimulus.dff_initial_xmask(s, 0.1, seed=8)
+    st = stimulus.build_states(s, cycles, seed=16, p=0.3, initial_xmask=m)
+    ref = st.copy()
+    sram = np.zeros(s.sram_storage_size, np.uint32)
+    sram_x = np.full(s.sram_storage_size, 0xFFFFFFFF, np.uint32)
+    ev_ref, n_ref = oracle.run(s, ref, cycles, sram=sram, sram_x=sram_x, constraints=cons,
+                               arrival_state_offset=s.arrival_state_offset,
+                               event_capacity=1 << 20)
+    out = sim.run_scan(s, st.copy(), cycles, constraints=cons, event_capacity=1 << 20)
+    _say("xts", s.num_parts, s.num_major_stages, "events", n_ref)
+    assert np.array_equal(out["states"], ref)
+    assert np.array_equal(out["sram"][:s.sram_storage_size], sram)
+    assert np.array_equal(out["sram_x"][:s.sram_storage_size], sram_x)
+    assert out["event_count"] == n_ref and n_ref > 0
+    assert np.array_equal(sim.canonical_events(out["events"]), sim.canonical_events(ev_ref))
