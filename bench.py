@@ -65,7 +65,10 @@ def parse():
     ap.add_argument("--batched-scale", type=float, default=0.02,
                     help="scale of the batched per-lane cycle counts "
                          "(1.0 = U[10K, 200K])")
-    ap.add_argument("--batched-blocks", type=int, default=0)
+    ap.add_argument("--batched-blocks", type=int, default=0,
+                    help="blocks per lane group (0 = autotune)")
+    ap.add_argument("--batched-groups", type=int, default=0,
+                    help="lane groups of 32 slots (0 = one per 32 lanes)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -175,20 +178,27 @@ def bench_plain(a, script, torch):
 
 
 def bench_batched(a, comp, torch):
-    from jacquard_amd import _native as N
-    from jacquard_amd.batched import BatchedRun
-    import ctypes
-    res = ctypes.c_int(0)
-    N.check(N.hip().jq_max_resident_blocks(1, ctypes.byref(res)))
-    groups = (a.lanes + 31) // 32
-    W = a.batched_blocks or max(1, min(64, res.value // groups))
-    script = comp.flatten(W)
+    """Batched form on the same netlist: --lanes lanes with per-lane seeds,
+    cycle counts U[10K, 200K] x --batched-scale and toggle rates
+    U[0.02, 0.5], in one persistent launch with the device lane queue.
+    Groups and blocks per group come from the autotuner unless given."""
+    from jacquard_amd import batched as B
     rng = np.random.default_rng(a.seed + 100)
     lo, hi = int(10_000 * a.batched_scale), int(200_000 * a.batched_scale)
     seeds = rng.integers(1, 2**31, a.lanes).astype(np.uint32)
     probs = rng.uniform(0.02, 0.5, a.lanes)
     cycles = rng.integers(max(lo, 1), max(hi, 1) + 1, a.lanes)
-    run = BatchedRun(script, seeds, probs, cycles)
+    tune = None
+    if a.batched_blocks:
+        W = a.batched_blocks
+        G = a.batched_groups or (a.lanes + 31) // 32
+        script = comp.flatten(W)
+    else:
+        t = time.time()
+        tune = B.autotune(comp, cycles, log=log)
+        W, G, script = tune["blocks"], tune["groups"], tune["script"]
+        log(f"autotune: chose {G} groups x {W} blocks in {time.time() - t:.1f}s")
+    run = B.BatchedRun(script, seeds, probs, cycles, groups=G)
     script.validate(script.reg_io_state_size)
     run.prepare()
     torch.cuda.synchronize()
@@ -203,24 +213,26 @@ def bench_batched(a, comp, torch):
     if device_error():
         raise RuntimeError("device barrier timeout during the batched run")
     lc = run.lane_cycles_total()
-    log(f"batched: {a.lanes} lanes x {groups} groups x {W} blocks, "
+    log(f"batched: {a.lanes} lanes through {G} groups x {W} blocks, "
         f"{lc} lane-cycles in {wall:.2f}s")
-    # spot-check one lane against the CPU oracle on its first cycles
+    # spot-check the first cycles of the shortest lane and of the last lane
+    # the queue handed out against the CPU oracle
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from jacquard_amd import stimulus
-    from jacquard_amd.batched import plain_observations
     outs = run.lane_outputs()
-    lane = int(np.argmin(cycles))
-    v = min(a.verify_cycles or 4, int(cycles[lane]))
-    st = stimulus.build_states(script, v, seed=int(seeds[lane]), p=float(probs[lane]))
-    oracle.run(script, st, v)
-    ok = np.array_equal(outs[lane][:v], plain_observations(script, st, v, run.obs_lits))
-    if not ok:
-        raise AssertionError("batched GPU lane differs from the CPU oracle")
+    checked = 0
+    for lane in {int(np.argmin(cycles)), int(run.order[-1])}:
+        v = min(a.verify_cycles or 4, int(cycles[lane]))
+        st = stimulus.build_states(script, v, seed=int(seeds[lane]), p=float(probs[lane]))
+        oracle.run(script, st, v)
+        if not np.array_equal(outs[lane][:v], B.plain_observations(script, st, v, run.obs_lits)):
+            raise AssertionError(f"batched GPU lane {lane} differs from the CPU oracle")
+        checked += v
     return dict(wall=wall, kernel_ms=e0.elapsed_time(e1), lane_cycles=lc,
-                slot_cycles=run.slot_cycles_total(), groups=groups, blocks=W,
-                verified_cycles=v)
+                slot_cycles=run.slot_cycles_total(), groups=G, blocks=W,
+                verified_cycles=checked, script=script,
+                tune=None if tune is None else tune["trials"])
 
 
 def bench_xtiming(a, torch):
@@ -460,7 +472,11 @@ def main():
                             "lanes": a.lanes, "groups": b["groups"],
                             "blocks_per_group": b["blocks"],
                             "lane_cycles": b["lane_cycles"],
+                            "cycles_per_lane": f"U[{int(10_000 * a.batched_scale)}, "
+                                               f"{int(200_000 * a.batched_scale)}]",
                             "slot_utilisation": b["lane_cycles"] / b["slot_cycles"],
+                            "kernel_s": b["kernel_ms"] / 1e3,
+                            "autotune": b["tune"],
                             "verified_cycles": b["verified_cycles"]}
         forms["batched"]["vs_plain"] = forms["batched"]["lane_cycles_per_s"] / value
         log(f"batched: {forms['batched']['lane_cycles_per_s']:,.0f} lane-cycles/s "

@@ -82,33 +82,55 @@ int jq_simulate_scan_profiled(usize num_blocks, usize num_major_stages,
                               u32 *samples, u32 sample_cycles, void *stream);
 
 /* ---- Batched lane-sliced form (no counterpart in the reference) ---------
- * Simulates 32*num_groups independent stimulus lanes of one script (value
- * lane: X masks and arrivals are not computed in this form) in a single
- * persistent launch.  State words are stored bit-transposed: for every
- * state bit, one u32 "lane-word" whose bit l belongs to lane l of a group.
+ * Simulates n_lanes independent stimulus lanes of one script (value lane:
+ * X masks and arrivals are not computed in this form) in a single
+ * persistent launch.  Lanes run 32 to a group: state words are stored
+ * bit-transposed, one u32 "lane-word" per state bit whose bit l belongs to
+ * the lane in slot l of the group.  Each group runs behind its own barrier;
+ * when a lane completes its cycles its slot takes the next lane of a device
+ * queue (its state bit cleared, clock flags set, its SRAM zeroed), so lanes
+ * of uneven length keep every slot busy.
  *   blocks_per_group : the script's num_blocks; the launch uses
  *                      num_groups * blocks_per_group co-resident workgroups
- *   group_cycles[g]  : cycles group g runs (<= max_cycles)
+ *   gather_table     : u32[num_parts * 8192], the script's global reads
+ *                      resolved per partition (blocks_data order): entry q
+ *                      is the state bit landing at first-layer position q
+ *                      (bit 31: read from the current snapshot) or
+ *                      0xFFFFFFFF for a zero position
+ *   gather_start     : u32[num_major_stages * blocks_per_group], index of
+ *                      the first partition of each (stage, block) in it
  *   states           : num_groups * 2 * (reg_io_state_size*32) u32: two
- *                      ping-pong snapshots per group, snapshot 0 in the
- *                      first (clock-flag lane-words pre-set to ~0 in both)
- *   sram_data        : num_groups * 32 * sram_storage_size u32 (per lane)
+ *                      ping-pong snapshots per group, both zero except the
+ *                      clock-flag lane-words (~0)
+ *   sram_data        : num_groups * 32 * sram_storage_size u32, zero
  *   data_bits[n_data]: state bit of each toggling primary input
- *   lane_seed/lane_thresh : per lane (32*num_groups): input bit k toggles at
- *                      cycle c when mix32(mix32(seed + 0x9E3779B9*k) ^ c) <
- *                      thresh, mix32 = the "lowbias32" mixer
- *   obs_bits[num_obs]: state bits captured after every cycle into
- *                      obs_out[(g*max_cycles + cycle)*num_obs + k]
+ *   flag_bits[n_flag]: state bit of each clock flag
+ *   lane_cycles/lane_seed/lane_thresh : per lane, in queue order; input bit
+ *                      k of a lane toggles before its cycle c (c >= 1) when
+ *                      mix32(mix32(seed + 0x9E3779B9*k) ^ c) < thresh,
+ *                      mix32 = the "lowbias32" mixer
+ *   slots            : num_groups * 32 u32, the lane of every slot at start
+ *                      (0xFFFFFFFF = empty); updated in place
+ *   queue            : one u32, the first lane not in a slot; updated
+ *   lane_info        : 2 u32 per lane, written: group*32 + slot, and the
+ *                      group cycle at which the lane started (pre-set by the
+ *                      caller for the lanes placed in slots)
+ *   group_cycles     : num_groups u32, written: cycles each group ran
+ *   obs_bits[num_obs]: state bits captured after every group cycle into
+ *                      obs_out[(g*obs_capacity + cycle)*num_obs + k]; lane
+ *                      (g, l, start) sees its cycle c at group cycle start+c
  * Returns 0 or a negative error code (message from jq_last_error()). */
 int jq_simulate_batched(usize num_groups, usize blocks_per_group,
                         usize num_major_stages, const usize *blocks_start,
-                        const u32 *blocks_data, u32 reg_io_state_size,
-                        u32 sram_storage_size, u32 *sram_data,
-                        const u32 *group_cycles, u32 max_cycles, u32 *states,
-                        u32 n_pi_words, const u32 *data_bits, u32 n_data,
+                        const u32 *blocks_data, const u32 *gather_table,
+                        const u32 *gather_start, u32 reg_io_state_size,
+                        u32 sram_storage_size, u32 *sram_data, u32 *states,
+                        const u32 *data_bits, u32 n_data, const u32 *flag_bits,
+                        u32 n_flag, u32 n_lanes, const u32 *lane_cycles,
                         const u32 *lane_seed, const u32 *lane_thresh,
-                        const u32 *obs_bits, u32 num_obs, u32 *obs_out,
-                        void *stream);
+                        u32 *lane_info, u32 *slots, u32 *queue,
+                        u32 *group_cycles, const u32 *obs_bits, u32 num_obs,
+                        u32 obs_capacity, u32 *obs_out, void *stream);
 
 /* Occupancy query used by the autotuner: the largest number of resident
  * workgroups (all co-resident, as the persistent launches require) for the

@@ -1,11 +1,13 @@
 """Batched lane-sliced form: many independent stimulus lanes of one netlist
 in a single persistent launch (jq_simulate_batched).
 
-Lanes are sorted by cycle count and packed 32 to a group, so the lanes of
-one group (which run in lockstep) have similar lengths; every group runs
-its own cycle count behind its own barrier.  Each lane is described by
-(seed, toggle probability, cycles) and sees exactly the stimulus that
-``stimulus.build_states(script, cycles, seed, p)`` gives the plain path.
+Lanes run 32 to a group (one bit of every state lane-word each); every
+group runs behind its own barrier and refills a slot from a device lane
+queue (longest lanes first) whenever the slot's lane completes.  Each lane
+is described by (seed, toggle probability, cycles) and sees exactly the
+stimulus that ``stimulus.build_states(script, cycles, seed, p)`` gives the
+plain path.  ``autotune`` picks the number of groups and the blocks per
+group from short measured trial launches.
 """
 import ctypes
 
@@ -15,38 +17,104 @@ from . import _native as N
 from . import stimulus
 
 
+NO_SOURCE = 0xFFFFFFFF
+
+
+def gather_table(script):
+    """Per-partition gather table of the batched form.
+
+    For partition p (blocks_data order, the rows of ``script.parts``) the
+    table holds 8192 u32: entry q is the state bit that lands at position q
+    of the first boomerang layer after the global read, with bit 31 set when
+    it is read from the current snapshot, or NO_SOURCE (position is zero).
+    This is the global read of the script format
+    (csrc/kernel_v1_impl.cuh:87-151, src/sim/cpu_reference.rs:47-71)
+    resolved once per script: thread t's (idx, mask) entries of rounds
+    0..R-1 are taken in order, each mask's set bits ascending, and the n-th
+    of its B_t bits lands at position 32*t + B_t - 1 - n.
+
+    Returns (table uint32[num_parts * 8192], start uint32[major * blocks]):
+    start[m*num_blocks + b] = index of the first partition of block b in
+    major stage m."""
+    bd = script.blocks_data
+    parts = np.asarray(script.parts)
+    n = len(parts)
+    table = np.full((max(n, 1), 8192), NO_SOURCE, np.uint32)
+    k32 = np.arange(32, dtype=np.uint32)
+    tt = np.arange(256, dtype=np.int64)
+    for p in range(n):
+        off = int(parts[p, 3])
+        R = int(bd[off + 6])
+        if R == 0:
+            continue
+        gr = bd[off + 256:off + 256 + 512 * R].reshape(R, 256, 2)
+        idx = gr[:, :, 0].T                     # [thread, round]
+        mask = gr[:, :, 1].T
+        bits = ((mask[:, :, None] >> k32) & 1).astype(bool)   # [t, r, k]
+        src = ((idx[:, :, None] & np.uint32(0x7FFFFFFF)) * np.uint32(32) + k32) \
+            | (idx[:, :, None] & np.uint32(0x80000000))
+        bits = bits.reshape(256, -1)
+        src = src.reshape(256, -1)
+        rank = np.cumsum(bits, axis=1) - 1
+        cnt = bits.sum(axis=1)
+        pos = 32 * tt[:, None] + cnt[:, None] - 1 - rank
+        table[p, pos[bits]] = src[bits]
+    start = np.zeros(script.num_major_stages * script.num_blocks, np.uint32)
+    seen = set()
+    for k in range(n):
+        key = int(parts[k, 0]) * script.num_blocks + int(parts[k, 1])
+        if key not in seen:
+            seen.add(key)
+            start[key] = k
+    return table.reshape(-1), start
+
+
 class BatchedRun:
-    def __init__(self, script, seeds, probs, cycles, obs_lits=None):
+    """Lanes of one script in one persistent launch.
+
+    Lanes enter a device queue longest first; the first groups*32 lanes are
+    placed in the slots (interleaved across groups, so every group starts
+    with a mix of long and short lanes) and each slot takes the next queued
+    lane when its lane completes.  `groups` defaults to one group per 32
+    lanes; fewer groups than that make the queue hand out the remaining
+    lanes as slots free up."""
+
+    def __init__(self, script, seeds, probs, cycles, obs_lits=None, groups=None):
         self.script = script
         self.seeds = np.asarray(seeds, np.uint32)
         self.probs = np.asarray(probs, np.float64)
         self.cycles = np.asarray(cycles, np.int64)
         n = len(self.seeds)
         assert len(self.probs) == n and len(self.cycles) == n
+        if n and int(self.cycles.min()) < 1:
+            raise ValueError("every lane needs at least one cycle")
+        if n and int(self.cycles.max()) >= 1 << 31:
+            raise ValueError("lane cycle counts must fit 31 bits")
         if obs_lits is None:
             obs_lits = [l for l in script.netlist.arrays()["pos"].tolist() if l > 1]
         self.obs_lits = list(obs_lits)
         self.obs_bits = np.asarray([script.output_map[l] for l in self.obs_lits], np.uint32)
-        # lane packing: longest first, 32 per group
-        self.order = np.argsort(-self.cycles, kind="stable")
-        self.G = (n + 31) // 32
-        slot_seed = np.zeros(self.G * 32, np.uint32)
-        slot_thr = np.zeros(self.G * 32, np.uint32)
-        self.gcycles = np.zeros(self.G, np.uint32)
-        for k, lane in enumerate(self.order.tolist()):
-            slot_seed[k] = self.seeds[lane]
-            slot_thr[k] = stimulus.threshold(self.probs[lane])
-            g = k // 32
-            self.gcycles[g] = max(self.gcycles[g], self.cycles[lane])
-        self.slot_seed, self.slot_thr = slot_seed, slot_thr
-        self.max_cycles = int(self.gcycles.max()) if n else 0
+        self.order = np.argsort(-self.cycles, kind="stable")  # queue order
+        self.G = groups or max(1, (n + 31) // 32)
+        self.q_cycles = self.cycles[self.order].astype(np.uint32)
+        self.q_seed = self.seeds[self.order]
+        self.q_thr = np.asarray([stimulus.threshold(self.probs[i]) for i in self.order.tolist()],
+                                np.uint32)
+        self.slots = np.full(self.G * 32, NO_SOURCE, np.uint32)
+        self.lane_info = np.zeros(2 * max(n, 1), np.uint32)
+        first = min(n, self.G * 32)
+        for k in range(first):
+            g, l = k % self.G, k // self.G
+            self.slots[g * 32 + l] = k
+            self.lane_info[2 * k] = g * 32 + l
+        self.first = first
+        total = int(self.cycles.sum())
+        self.max_cycles = int(self.cycles.max()) if n else 0
+        self.obs_cap = -(-total // (32 * self.G)) + self.max_cycles + 1
         self.data_bits, self.flag_bits = stimulus.input_roles(script)
 
     def lane_cycles_total(self):
         return int(self.cycles.sum())
-
-    def slot_cycles_total(self):
-        return int(self.gcycles.sum()) * 32
 
     def prepare(self):
         """Allocate and initialise the device buffers."""
@@ -56,18 +124,27 @@ class BatchedRun:
         rio = s.reg_io_state_size
         st = np.zeros((self.G, 2, rio * 32), np.uint32)
         st[:, :, self.flag_bits.astype(np.int64)] = 0xFFFFFFFF
+        table, start = gather_table(s)
+        nz = lambda a: a if a.size else np.zeros(1, a.dtype)
         self.d = dict(
             bs=to_device(np.ascontiguousarray(s.blocks_start, np.uint64)),
             bd=to_device(np.ascontiguousarray(s.blocks_data, np.uint32)),
+            gtab=to_device(table),
+            gstart=to_device(start),
             states=to_device(st.reshape(-1)),
             sram=torch.zeros(max(self.G * 32 * s.sram_storage_size, 1),
                              dtype=torch.int32, device="cuda"),
-            gcycles=to_device(self.gcycles),
-            data_bits=to_device(self.data_bits),
-            seed=to_device(self.slot_seed),
-            thr=to_device(self.slot_thr),
-            obs_bits=to_device(self.obs_bits),
-            obs=torch.zeros(max(self.G * self.max_cycles * len(self.obs_bits), 1),
+            data_bits=to_device(nz(self.data_bits)),
+            flag_bits=to_device(nz(self.flag_bits)),
+            lcyc=to_device(nz(self.q_cycles)),
+            seed=to_device(nz(self.q_seed)),
+            thr=to_device(nz(self.q_thr)),
+            info=to_device(self.lane_info),
+            slots=to_device(self.slots),
+            queue=to_device(np.asarray([self.first], np.uint32)),
+            gcyc=torch.zeros(self.G, dtype=torch.int32, device="cuda"),
+            obs_bits=to_device(nz(self.obs_bits)),
+            obs=torch.zeros(max(self.G * self.obs_cap * len(self.obs_bits), 1),
                             dtype=torch.int32, device="cuda"),
         )
         return self.d
@@ -79,12 +156,14 @@ class BatchedRun:
         st = stream if stream is not None else torch.cuda.current_stream()
         rc = N.hip().jq_simulate_batched(
             self.G, s.num_blocks, s.num_major_stages, d["bs"].data_ptr(),
-            d["bd"].data_ptr(), s.reg_io_state_size, s.sram_storage_size,
-            d["sram"].data_ptr(), d["gcycles"].data_ptr(), self.max_cycles,
-            d["states"].data_ptr(), stimulus.input_word_count(s),
-            d["data_bits"].data_ptr(), len(self.data_bits), d["seed"].data_ptr(),
-            d["thr"].data_ptr(), d["obs_bits"].data_ptr(), len(self.obs_bits),
-            d["obs"].data_ptr(), ctypes.c_void_p(st.cuda_stream))
+            d["bd"].data_ptr(), d["gtab"].data_ptr(), d["gstart"].data_ptr(),
+            s.reg_io_state_size, s.sram_storage_size, d["sram"].data_ptr(),
+            d["states"].data_ptr(), d["data_bits"].data_ptr(), len(self.data_bits),
+            d["flag_bits"].data_ptr(), len(self.flag_bits), len(self.seeds),
+            d["lcyc"].data_ptr(), d["seed"].data_ptr(), d["thr"].data_ptr(),
+            d["info"].data_ptr(), d["slots"].data_ptr(), d["queue"].data_ptr(),
+            d["gcyc"].data_ptr(), d["obs_bits"].data_ptr(), len(self.obs_bits),
+            self.obs_cap, d["obs"].data_ptr(), ctypes.c_void_p(st.cuda_stream))
         N.check(rc)
 
     def run(self, validate=True):
@@ -100,16 +179,25 @@ class BatchedRun:
             raise RuntimeError("device barrier timed out in the batched launch")
         return self.lane_outputs()
 
+    def group_cycles(self):
+        return self.d["gcyc"].cpu().numpy().astype(np.int64)
+
+    def slot_cycles_total(self):
+        """Slot-cycles the groups ran (32 per group cycle)."""
+        return int(self.group_cycles().sum()) * 32
+
     def lane_outputs(self):
         """{lane: uint8[cycles, n_obs]} observed values after every cycle."""
+        n_obs = len(self.obs_bits)
         obs = self.d["obs"].cpu().numpy().view(np.uint32)
-        obs = obs[:self.G * self.max_cycles * len(self.obs_bits)].reshape(
-            self.G, self.max_cycles, len(self.obs_bits))
+        obs = obs[:self.G * self.obs_cap * n_obs].reshape(self.G, self.obs_cap, n_obs)
+        info = self.d["info"].cpu().numpy().view(np.uint32)
         out = {}
         for k, lane in enumerate(self.order.tolist()):
-            g, l = divmod(k, 32)
+            gs, start = int(info[2 * k]), int(info[2 * k + 1])
+            g, l = divmod(gs, 32)
             c = int(self.cycles[lane])
-            out[lane] = ((obs[g, :c, :] >> np.uint32(l)) & 1).astype(np.uint8)
+            out[lane] = ((obs[g, start:start + c, :] >> np.uint32(l)) & 1).astype(np.uint8)
         return out
 
 
@@ -122,3 +210,70 @@ def plain_observations(script, states, cycles, obs_lits):
         pos = script.output_map[lit]
         cols.append((S[1:cycles + 1, pos >> 5] >> np.uint32(pos & 31)) & 1)
     return np.stack(cols, 1).astype(np.uint8)
+
+
+def max_resident_batched():
+    """Co-resident workgroups of the batched kernel on this device."""
+    r = ctypes.c_int(0)
+    N.check(N.hip().jq_max_resident_blocks(1, ctypes.byref(r)))
+    return r.value
+
+
+def predicted_makespan_cycles(lane_cycles, groups):
+    """Group cycles until the last lane completes with `groups`*32 slots fed
+    longest-first: every slot is busy until the queue runs dry, so the
+    makespan is bounded by max(total / slots, longest lane) (+ the tail of
+    the last lanes, which the longest lane dominates)."""
+    lane_cycles = np.asarray(lane_cycles, np.int64)
+    return max(-(-int(lane_cycles.sum()) // (32 * groups)), int(lane_cycles.max()))
+
+
+def autotune(compiled, lane_cycles, widths=(16, 24, 32, 48, 64), trial_cycles=24,
+             log=None):
+    """Choose (groups, blocks per group) for a lane set.
+
+    For every candidate width W (blocks per group, i.e. the script is
+    flattened for W blocks) and group count G that fits co-resident, one
+    trial launch of G full groups runs `trial_cycles` cycles; its kernel time
+    per cycle times the predicted makespan in cycles ranks the candidates.
+    Returns dict(groups, blocks, script, us_per_cycle, trials)."""
+    import torch
+    resident = max_resident_batched()
+    n = len(lane_cycles)
+    g_full = max(1, -(-n // 32))
+    trials = []
+    best = None
+    rng = np.random.default_rng(0)
+    for W in widths:
+        if W > resident:
+            continue
+        s = compiled.flatten(W)
+        gs = sorted({max(1, min(g_full, resident // W, g_full >> k)) for k in range(4)},
+                    reverse=True)
+        for G in gs:
+            if G * W > resident:
+                continue
+            cyc = np.full(32 * G, trial_cycles)
+            run = BatchedRun(s, rng.integers(1, 2**31, 32 * G), np.full(32 * G, 0.1), cyc,
+                             obs_lits=[], groups=G)
+            run.prepare()
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            run.launch()
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / trial_cycles
+            span = predicted_makespan_cycles(lane_cycles, G) * us
+            trials.append(dict(blocks=W, groups=G, us_per_cycle=us, makespan_us=span))
+            if log:
+                log(f"autotune: W={W} G={G}: {us:.1f} us/cycle, predicted makespan "
+                    f"{span / 1e6:.2f} s")
+            if best is None or span < best[0]:
+                best = (span, W, G, s, us)
+            del run
+    if best is None:
+        raise RuntimeError("autotune: no candidate fits the device")
+    return dict(groups=best[2], blocks=best[1], script=best[3], us_per_cycle=best[4],
+                trials=trials)

@@ -9,11 +9,21 @@
 // gathers per boomerang stage become 8192 word gathers shared by 32 lanes.
 //
 // Work mapping inside a workgroup (256 threads, 4 x wave64):
+//   - global read: a per-partition gather table (built once per script from
+//     its (idx, mask) rounds) gives the source lane-word of each of the 8192
+//     first-layer positions, so each thread issues 32 independent loads
+//     instead of a chain of dependent pext steps;
+//   - every 5120-word script section (boomerang stage, SRAM/dup, clock
+//     enable) is copied into LDS with coalesced 16-byte loads, one section
+//     ahead of its use (double buffer), so the per-position permutation and
+//     flag reads are LDS reads;
 //   - thread t owns level-1 outputs t + 256k (k < 16), so levels 1..5 of its
 //     subtree stay in registers; wave 0 finishes levels 6..13 with
 //     cross-lane moves: two workgroup barriers per boomerang stage;
-//   - LDS (dynamic, 96 KiB): two after-layouts (ping-pong) and the 256-word
-//     writeout area, each 8192 lane-words;
+//   - the three 8192-word LDS arrays (two after-layouts, the writeout area)
+//     are row-swizzled: word w of 32-word row r lives at w ^ (r & 31), so
+//     accesses where each lane touches the same word of a different row
+//     spread over all banks;
 //   - one persistent workgroup per (lane group, script block); each lane
 //     group has its own barrier among its blocks, so groups with different
 //     cycle counts run independently;
@@ -32,26 +42,38 @@ __device__ u32 gb_error;
 
 namespace {
 
-constexpr int kLdsWords = 3 * 8192;
+constexpr int kLayerWords = 8192;
+constexpr int kSecWords = 5120;  // 4096 permutation words + 256 x 4 flags
+constexpr int kLdsWords = 3 * kLayerWords + 2 * kSecWords;
+constexpr u32 kNoSource = 0xFFFFFFFFu;
 
 struct BArgs {
   u32 G, W, num_major;
   const usize *bs;
   const u32 *bd;
+  const u32 *gsrc;   // gather table: 8192 u32 per partition (blocks_data order)
+  const u32 *gidx;   // first partition index of each (major stage, block)
   u32 rio, sram_size;
   u32 *sram;
-  const u32 *gcycles;
-  u32 max_cycles;
   u32 *states;
-  u32 n_pi_words;
   const u32 *data_bits;
   u32 n_data;
-  const u32 *lane_seed, *lane_thresh;
+  const u32 *flag_bits;
+  u32 n_flag;
+  // lane queue (lanes in queue order)
+  u32 n_lanes;
+  const u32 *lane_cycles, *lane_seed, *lane_thresh;
+  u32 *lane_info;    // per lane: group*32 + slot, start cycle of the group
+  u32 *slots;        // per group 32 slots: lane or kNoLane
+  u32 *queue;        // next lane to hand out
+  u32 *group_cycles; // out: cycles each group ran
   const u32 *obs_bits;
-  u32 n_obs;
+  u32 n_obs, obs_cap;
   u32 *obs_out;
   u32 *bar;
 };
+
+constexpr u32 kNoLane = 0xFFFFFFFFu;
 
 __device__ __forceinline__ u32 mix32(u32 x) {
   x ^= x >> 16;
@@ -61,6 +83,9 @@ __device__ __forceinline__ u32 mix32(u32 x) {
   x ^= x >> 16;
   return x;
 }
+
+// Row swizzle of the 8192-word LDS arrays.
+__device__ __forceinline__ u32 sw(u32 p) { return p ^ ((p >> 5) & 31u); }
 
 // u16 permutation entry of bit position p of a 4x1024-word section
 // (src/flatten.rs:1027-1034).
@@ -88,14 +113,34 @@ __device__ __forceinline__ u32 lgate(u32 a, u32 b, Flags f) {
   return (a ^ f.xa) & ((b ^ f.xb) | f.ob);
 }
 
-__device__ void run_block_lanes(const BArgs &A, const u32 *__restrict__ script,
+// Copy one 5120-word script section from global memory straight into LDS
+// (global_load_lds_dwordx4: no registers, completes asynchronously).  Lane
+// l of wave w moves the 16 bytes at word 4*(t + 256j), t = 64w + l, to the
+// same word offset in LDS; the LDS base of each instruction is wave-uniform.
+__device__ __forceinline__ void sec_fetch(const u32 *g, u32 *l) {
+  const u32 t = threadIdx.x, w = t >> 6;
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+    __builtin_amdgcn_global_load_lds(g + 4 * (t + 256 * j), l + 4 * (64 * w + 256 * j),
+                                     16, 0, 0);
+}
+
+// Wait until this wave's outstanding global loads (including the LDS
+// copies above) have landed: s_waitcnt vmcnt(0).  Every wave runs it before
+// the workgroup barrier that precedes the first read of a fetched section.
+__device__ __forceinline__ void sec_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+__device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__restrict__ script,
+                                const u32 *__restrict__ gsrc,
                                 const u32 *__restrict__ in, u32 *__restrict__ out,
                                 u32 *ram_group, u32 *lds) {
   const u32 t = threadIdx.x;
   const u32 lane = t & 63;
-  u32 *bufA = lds, *bufB = lds + 8192, *WO = lds + 16384;
+  u32 *bufA = lds, *bufB = lds + kLayerWords, *WO = lds + 2 * kLayerWords;
+  // section buffer k & 1 (pointer arithmetic, not an array: no scratch)
+  auto secb = [lds](u32 k) { return lds + 3 * kLayerWords + (k & 1u) * kSecWords; };
   size_t pi = 0;
-  for (;;) {
+  for (;; gsrc += kLayerWords) {
     const u32 S = script[pi];
     if (S == 0) break;
     const u32 is_last = script[pi + 1];
@@ -107,98 +152,94 @@ __device__ void run_block_lanes(const BArgs &A, const u32 *__restrict__ script,
     const u32 nd = script[pi + 7];
     u32 hook = script[pi + 128 + (t >> 1)];
     hook = (t & 1) ? (hook >> 16) : (hook & 0xffffu);
-    pi += 256;
+    // sections: stage 0..S-1, SRAM/dup, clock enable, contiguous
+    const u32 *secg = script + pi + 256 + (size_t)512 * R;
+    pi += 256 + (size_t)512 * R + (size_t)kSecWords * (S + 2);
 
-    // ---- global read: thread t fills positions 32t .. 32t+31 in the
-    // packed order (first extracted bit lands highest), zero above
+    sec_fetch(secg, secb(0));
+    // ---- global read through the gather table: 32 independent loads
     {
-      const uint2 *gr = reinterpret_cast<const uint2 *>(script + pi);
-      u32 B = 0;
-      for (u32 r = 0; r < R; ++r) B += __popc(gr[256 * r + t].y);
-      u32 pos = B;
-      for (u32 r = 0; r < R; ++r) {
-        uint2 e = gr[256 * r + t];
-        if (!e.y) continue;
-        const u32 *base = (e.x >> 31) ? out : in;
-        const u32 *wbase = base + (size_t)(e.x & 0x7fffffffu) * 32;
-        u32 m = e.y;
-        while (m) {
-          u32 k = __ffs(m) - 1;
-          m &= m - 1;
-          --pos;
-          bufA[32 * t + pos] = wbase[k];
-        }
+      u32 src[32];
+#pragma unroll
+      for (int j = 0; j < 32; ++j) src[j] = gsrc[t + 256 * j];
+#pragma unroll
+      for (int j = 0; j < 32; ++j) {
+        u32 e = src[j], v = 0;
+        if (e != kNoSource) v = ((e >> 31) ? out : in)[e & 0x7fffffffu];
+        bufA[sw(t + 256 * j)] = v;
       }
-      for (u32 q = B; q < 32; ++q) bufA[32 * t + q] = 0;
-      pi += (size_t)512 * R;
     }
+    sec_wait();
     __syncthreads();
 
     u32 *prev = bufA, *cur = bufB;
     for (u32 s = 0; s < S; ++s) {
-      const u32 *sec = script + pi;
+      const u32 *sec = secb(s);
       const u32 *fl = sec + 4096;
-      pi += 5120;
+      // the next section goes into the other buffer, last read by the
+      // previous stage, which finished before the last barrier
+      sec_fetch(secg + (size_t)kSecWords * (s + 1), secb(s + 1));
       // levels 1..5 of the subtree rooted at position 256 + t
       u32 L1[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
         u32 j = t + 256u * k;
-        u32 a = prev[perm_at(sec, j)];
-        u32 b = prev[perm_at(sec, j + 4096)];
+        u32 a = prev[sw(perm_at(sec, j))];
+        u32 b = prev[sw(perm_at(sec, j + 4096))];
         L1[k] = lgate(a, b, flags_at(fl, 4096 + j));
-        cur[4096 + j] = L1[k];
+        cur[sw(4096 + j)] = L1[k];
       }
       u32 L2[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         u32 p = 2048 + t + 256u * k;
         L2[k] = lgate(L1[k], L1[k + 8], flags_at(fl, p));
-        cur[p] = L2[k];
+        cur[sw(p)] = L2[k];
       }
       u32 L3[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         u32 p = 1024 + t + 256u * k;
         L3[k] = lgate(L2[k], L2[k + 4], flags_at(fl, p));
-        cur[p] = L3[k];
+        cur[sw(p)] = L3[k];
       }
       u32 L4[2];
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         u32 p = 512 + t + 256u * k;
         L4[k] = lgate(L3[k], L3[k + 2], flags_at(fl, p));
-        cur[p] = L4[k];
+        cur[sw(p)] = L4[k];
       }
-      cur[256 + t] = lgate(L4[0], L4[1], flags_at(fl, 256 + t));
+      cur[sw(256 + t)] = lgate(L4[0], L4[1], flags_at(fl, 256 + t));
       __syncthreads();
       if (t < 64) {
         // level 6: positions 128+L and 192+L; level 7: 64+L
-        u32 o6a = lgate(cur[256 + lane], cur[384 + lane], flags_at(fl, 128 + lane));
-        u32 o6b = lgate(cur[320 + lane], cur[448 + lane], flags_at(fl, 192 + lane));
+        u32 o6a = lgate(cur[sw(256 + lane)], cur[sw(384 + lane)], flags_at(fl, 128 + lane));
+        u32 o6b = lgate(cur[sw(320 + lane)], cur[sw(448 + lane)], flags_at(fl, 192 + lane));
         u32 o7 = lgate(o6a, o6b, flags_at(fl, 64 + lane));
         // level 8: lanes 32..63 -> position L from L+32 (lane L-32) and
         // L+64 (own level-7 word)
         u32 c = 0;
+        const Flags f0 = flags_at(fl, lane);
         u32 a8 = __shfl(o7, (lane + 32) & 63);
-        if (lane >= 32) c = lgate(a8, o7, flags_at(fl, lane));
+        if (lane >= 32) c = lgate(a8, o7, f0);
 #pragma unroll
         for (int len = 16; len >= 1; len >>= 1) {
           u32 a = __shfl(c, (lane + len) & 63);
           u32 b = __shfl(c, (lane + 2 * len) & 63);
-          if (lane >= (u32)len && lane < 2u * len) c = lgate(a, b, flags_at(fl, lane));
+          if (lane >= (u32)len && lane < 2u * len) c = lgate(a, b, f0);
         }
-        cur[128 + lane] = o6a;
-        cur[192 + lane] = o6b;
-        cur[64 + lane] = o7;
-        cur[lane] = c;  // lane 0 keeps position 0 at zero
+        cur[sw(128 + lane)] = o6a;
+        cur[sw(192 + lane)] = o6b;
+        cur[sw(64 + lane)] = o7;
+        cur[sw(lane)] = c;  // lane 0 keeps position 0 at zero
       }
+      sec_wait();  // section s+1 is read right after this barrier
       __syncthreads();
       if ((hook >> 8) == s) {
-        const uint4 *src = reinterpret_cast<const uint4 *>(cur + 32 * (hook & 255));
-        uint4 *dst = reinterpret_cast<uint4 *>(WO + 32 * t);
+        const u32 src = 32u * (hook & 255u);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) dst[q] = src[q];
+        for (u32 q = 0; q < 32; ++q) WO[sw(32 * t + q)] = cur[sw(src + q)];
       }
       u32 *tmp = prev;
       prev = cur;
@@ -209,14 +250,15 @@ __device__ void run_block_lanes(const BArgs &A, const u32 *__restrict__ script,
     // ---- SRAM + duplicate permutation into bufA (free now)
     const u32 nperm = ns * 4 + nd;
     {
-      const u32 *sec = script + pi;
+      const u32 *sec = secb(S);
       const u32 *fl = sec + 4096;
+      // the clock-enable section goes into the buffer of the last stage
+      sec_fetch(secg + (size_t)kSecWords * (S + 1), secb(S + 1));
       for (u32 q = t; q < nperm * 32; q += 256) {
         uint4 f = *reinterpret_cast<const uint4 *>(fl + 4 * (q >> 5));
-        u32 v = WO[perm_at(sec, q)];
-        bufA[q] = (v & ~fbit(f.y, q)) ^ fbit(f.x, q);
+        u32 v = WO[sw(perm_at(sec, q))];
+        bufA[sw(q)] = (v & ~fbit(f.y, q)) ^ fbit(f.x, q);
       }
-      pi += 5120;
     }
     __syncthreads();
     // SRAM ports: thread per (sram, lane)
@@ -226,15 +268,15 @@ __device__ void run_block_lanes(const BArgs &A, const u32 *__restrict__ script,
       u32 r = 0;
       bool act = id < ns * 32;
       if (act) {
-        const u32 *P = bufA + 128 * s;
+        const u32 P = 128 * s;
         u32 ra = 0, wa = 0, en = 0, data = 0;
         for (int k = 0; k < 13; ++k) {
-          ra |= ((P[k] >> l) & 1u) << k;
-          wa |= ((P[16 + k] >> l) & 1u) << k;
+          ra |= ((bufA[sw(P + k)] >> l) & 1u) << k;
+          wa |= ((bufA[sw(P + 16 + k)] >> l) & 1u) << k;
         }
         for (int k = 0; k < 32; ++k) {
-          en |= ((P[32 + k] >> l) & 1u) << k;
-          data |= ((P[64 + k] >> l) & 1u) << k;
+          en |= ((bufA[sw(P + 32 + k)] >> l) & 1u) << k;
+          data |= ((bufA[sw(P + 64 + k)] >> l) & 1u) << k;
         }
         u32 *ram = ram_group + (size_t)l * A.sram_size + so + (size_t)s * kSramWords;
         r = ram[ra];
@@ -248,25 +290,25 @@ __device__ void run_block_lanes(const BArgs &A, const u32 *__restrict__ script,
         u32 half = (u32)(m >> (32 * ((t >> 5) & 1)));
         if (l == (u32)k) mine = half;
       }
-      if (act) WO[32 * (num_ios - ns + s) + l] = mine;
+      if (act) WO[sw(32 * (num_ios - ns + s) + l)] = mine;
     }
     for (u32 q = t; q < nd * 32; q += 256)
-      WO[32 * (num_ios - ns - nd) + q] = bufA[ns * 128 + q];
+      WO[sw(32 * (num_ios - ns - nd) + q)] = bufA[sw(ns * 128 + q)];
+    sec_wait();  // the clock-enable section is read after this barrier
     __syncthreads();
 
     // ---- clock-enable permutation + commit (coalesced lane-words)
     {
-      const u32 *sec = script + pi;
+      const u32 *sec = secb(S + 1);
       const u32 *fl = sec + 4096;
       u32 *o = out + (size_t)io * 32;
       const u32 *i0 = in + (size_t)io * 32;
       for (u32 q = t; q < num_ios * 32; q += 256) {
         uint4 f = *reinterpret_cast<const uint4 *>(fl + 4 * (q >> 5));
-        u32 ck = (WO[perm_at(sec, q)] & ~fbit(f.y, q)) ^ fbit(f.x, q);
-        u32 d = WO[q] ^ fbit(f.z, q);
+        u32 ck = (WO[sw(perm_at(sec, q))] & ~fbit(f.y, q)) ^ fbit(f.x, q);
+        u32 d = WO[sw(q)] ^ fbit(f.z, q);
         o[q] = (i0[q] & ~ck) | (d & ck);
       }
-      pi += 5120;
     }
     __syncthreads();
     if (is_last) break;
@@ -278,49 +320,96 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
   const u32 g = blockIdx.x / A.W, b = blockIdx.x % A.W;
   u32 *bar = A.bar + kSlotWords * g;
   u32 gen = read_gen(bar);
-  const u32 ncyc = A.gcycles[g];
   const size_t snap = (size_t)A.rio * 32;
   u32 *st0 = A.states + (size_t)g * 2 * snap;
   u32 *st1 = st0 + snap;
   u32 *ram_group = A.sram + (size_t)g * 32 * A.sram_size;
-  const u32 t = threadIdx.x;
+  u32 *slots = A.slots + 32 * g;
+  const u32 t = threadIdx.x, l = t & 31;
+  const u32 par = b * 256 + t, npar = A.W * 256;  // group-wide thread index
+  // every thread mirrors slot l = t & 31 of its group
+  u32 my_lane = slots[l], my_start = 0, my_end = 0, my_seed = 0, my_thr = 0;
+  if (my_lane != kNoLane) {
+    my_end = A.lane_cycles[my_lane];
+    my_seed = A.lane_seed[my_lane];
+    my_thr = A.lane_thresh[my_lane];
+  }
   __syncthreads();
-  for (u32 c = 0; c < ncyc; ++c) {
-    const u32 *in = (c & 1) ? st1 : st0;
+  u32 c = 0;
+  for (;; ++c) {
+    u32 *in = (c & 1) ? st1 : st0;
     u32 *out = (c & 1) ? st0 : st1;
-    if (b == 0 && c > 0) {
-      u32 *o = A.obs_out + ((size_t)g * A.max_cycles + (c - 1)) * A.n_obs;
+    // every group finishes within obs_cap cycles (queue bound); a launch
+    // whose lane table breaks that bound stops here with an error instead
+    // of looping (the same decision in every block of the group)
+    if (c > A.obs_cap) {
+      if (b == 0 && t == 0) atomicOr(&gb_error, 2u);
+      break;
+    }
+    if (b == 0 && c > 0 && c - 1 < A.obs_cap) {
+      u32 *o = A.obs_out + ((size_t)g * A.obs_cap + (c - 1)) * A.n_obs;
       for (u32 k = t; k < A.n_obs; k += 256) o[k] = in[A.obs_bits[k]];
     }
-    // stimulus of snapshot c+1: data bit k toggles for lane l when
-    // hash(seed_l, c+1, k) < thresh_l
+    // lanes that completed their cycles hand their slot to the next lane
+    const bool fin = my_lane != kNoLane && c >= my_end;
+    if (__syncthreads_or(fin)) {
+      const u32 finmask = (u32)__ballot(fin);  // bit l: slot l finished
+      if (b == 0 && t < 32 && fin) {
+        u32 nl = atomicAdd(A.queue, 1u);
+        if (nl < A.n_lanes) {
+          slots[l] = nl;
+          A.lane_info[2 * nl] = g * 32 + l;
+          A.lane_info[2 * nl + 1] = c;
+        } else {
+          slots[l] = kNoLane;
+        }
+      }
+      grid_sync(bar, A.W, gen, &gb_error);
+      // reset the finished slots' lane bit: state of this cycle's input,
+      // clock flags set, the lane's SRAM words zeroed
+      for (size_t w = par; w < snap; w += npar) in[w] &= ~finmask;
+      for (u32 k = par; k < A.n_flag; k += npar) in[A.flag_bits[k]] |= finmask;
+      for (u32 m = finmask; m; m &= m - 1) {
+        u32 *ram = ram_group + (size_t)(__ffs(m) - 1) * A.sram_size;
+        for (u32 w = par; w < A.sram_size; w += npar) ram[w] = 0;
+      }
+      if (fin) {
+        my_lane = __hip_atomic_load(&slots[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        my_start = c;
+        if (my_lane != kNoLane) {
+          my_end = c + A.lane_cycles[my_lane];
+          my_seed = A.lane_seed[my_lane];
+          my_thr = A.lane_thresh[my_lane];
+        }
+      }
+      grid_sync(bar, A.W, gen, &gb_error);
+    }
+    if (!__syncthreads_or(my_lane != kNoLane)) break;
+    // stimulus of snapshot c+1: data bit k toggles for the lane in slot l
+    // when hash(seed, lane cycle + 1, k) < thresh
     for (u32 k0 = b * 8; k0 < A.n_data; k0 += A.W * 8) {
-      u32 k = k0 + (t >> 5), l = t & 31;
-      bool act = k < A.n_data;
+      u32 k = k0 + (t >> 5);
+      bool act = k < A.n_data && my_lane != kNoLane;
       u32 bit = 0;
       if (act) {
-        u32 gl = g * 32 + l;
-        u32 h = mix32(mix32(A.lane_seed[gl] + 0x9E3779B9u * k) ^ (c + 1));
-        bit = h < A.lane_thresh[gl];
+        u32 h = mix32(mix32(my_seed + 0x9E3779B9u * k) ^ (c - my_start + 1));
+        bit = h < my_thr;
       }
       unsigned long long m = __ballot(act && bit);
       u32 tog = (u32)(m >> (32 * ((t >> 5) & 1)));
-      if (act && l == 0) {
+      if (k < A.n_data && l == 0) {
         u32 p = A.data_bits[k];
         out[p] = in[p] ^ tog;
       }
     }
     for (u32 mj = 0; mj < A.num_major; ++mj) {
-      usize s0 = A.bs[(size_t)mj * A.W + b];
-      run_block_lanes(A, A.bd + s0, in, out, ram_group, lds);
+      const size_t k = (size_t)mj * A.W + b;
+      run_block_lanes(A, A.bd + A.bs[k], A.gsrc + (size_t)A.gidx[k] * kLayerWords, in,
+                      out, ram_group, lds);
       grid_sync(bar, A.W, gen, &gb_error);
     }
   }
-  if (b == 0 && ncyc > 0) {
-    const u32 *fin = (ncyc & 1) ? st1 : st0;
-    u32 *o = A.obs_out + ((size_t)g * A.max_cycles + (ncyc - 1)) * A.n_obs;
-    for (u32 k = t; k < A.n_obs; k += 256) o[k] = fin[A.obs_bits[k]];
-  }
+  if (b == 0 && t == 0) A.group_cycles[g] = c;
 }
 
 u32 b_slot = 0;
@@ -362,15 +451,21 @@ using namespace jqd;
 
 extern "C" int jq_simulate_batched(
     usize num_groups, usize blocks_per_group, usize num_major_stages,
-    const usize *blocks_start, const u32 *blocks_data, u32 reg_io_state_size,
-    u32 sram_storage_size, u32 *sram_data, const u32 *group_cycles,
-    u32 max_cycles, u32 *states, u32 n_pi_words, const u32 *data_bits,
-    u32 n_data, const u32 *lane_seed, const u32 *lane_thresh,
-    const u32 *obs_bits, u32 num_obs, u32 *obs_out, void *stream) {
+    const usize *blocks_start, const u32 *blocks_data, const u32 *gather_table,
+    const u32 *gather_start, u32 reg_io_state_size, u32 sram_storage_size,
+    u32 *sram_data, u32 *states, const u32 *data_bits, u32 n_data,
+    const u32 *flag_bits, u32 n_flag, u32 n_lanes, const u32 *lane_cycles,
+    const u32 *lane_seed, const u32 *lane_thresh, u32 *lane_info, u32 *slots,
+    u32 *queue, u32 *group_cycles, const u32 *obs_bits, u32 num_obs,
+    u32 obs_capacity, u32 *obs_out, void *stream) {
   set_error("");
   if (num_groups == 0 || blocks_per_group == 0) return 0;
   if (num_groups > (usize)kBarrierSlots) {
     return set_error("too many lane groups");
+  }
+  if (!gather_table || !gather_start || !slots || !queue || !lane_info ||
+      !group_cycles) {
+    return set_error("batched launch: missing gather table or lane-queue buffers");
   }
   int per = 0;
   if (batched_blocks_per_cu(&per)) {
@@ -382,19 +477,27 @@ extern "C" int jq_simulate_batched(
   A.num_major = (u32)num_major_stages;
   A.bs = blocks_start;
   A.bd = blocks_data;
+  A.gsrc = gather_table;
+  A.gidx = gather_start;
   A.rio = reg_io_state_size;
   A.sram_size = sram_storage_size;
   A.sram = sram_data;
-  A.gcycles = group_cycles;
-  A.max_cycles = max_cycles;
   A.states = states;
-  A.n_pi_words = n_pi_words;
   A.data_bits = data_bits;
   A.n_data = n_data;
+  A.flag_bits = flag_bits;
+  A.n_flag = n_flag;
+  A.n_lanes = n_lanes;
+  A.lane_cycles = lane_cycles;
   A.lane_seed = lane_seed;
   A.lane_thresh = lane_thresh;
+  A.lane_info = lane_info;
+  A.slots = slots;
+  A.queue = queue;
+  A.group_cycles = group_cycles;
   A.obs_bits = obs_bits;
   A.n_obs = num_obs;
+  A.obs_cap = obs_capacity;
   A.obs_out = obs_out;
   u32 *bar_base = nullptr;
   if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(gb_barrier)) != hipSuccess) {

@@ -38,17 +38,22 @@ constexpr int kSlotWords = 64;
 __device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen, u32 *err) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    // the arrival is an agent-scope acq_rel RMW (releases this workgroup's
-    // stores, ordered after them by the barrier above); the waiters'
-    // acquire loads make every other workgroup's stores visible
-    u32 arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL,
+    // One agent-scope release fence publishes this workgroup's stores
+    // (ordered after them by the barrier above), and one acquire fence after
+    // the wait makes the other workgroups' stores visible.  The arrival and
+    // the polling are relaxed agent-scope atomics: on gfx950 an acquire on
+    // every poll would invalidate the XCD's L2 once per poll, for every
+    // waiting workgroup.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    u32 arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == n - 1) {
       __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // release: the counter reset is visible before the new generation
       __hip_atomic_fetch_add(&bar[32], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       u32 spins = 0;
-      while (__hip_atomic_load(&bar[32], __ATOMIC_ACQUIRE,
+      while (__hip_atomic_load(&bar[32], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT) == gen) {
         __builtin_amdgcn_s_sleep(1);
         if (++spins > (1u << 27)) {
@@ -57,6 +62,7 @@ __device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen, u32 *err) {
         }
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     gen += 1;
   }
   __syncthreads();

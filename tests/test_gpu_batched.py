@@ -1,6 +1,8 @@
 """Parity of the batched lane-sliced kernel with per-lane CPU oracle runs:
 every observed bit of every lane and cycle must equal the oracle's run of
-that lane's stimulus through the plain (packed) semantics."""
+that lane's stimulus through the plain (packed) semantics.  Runs with fewer
+slots than lanes exercise the device lane queue: a slot that finishes a
+lane is reset (state bit, clock flags, SRAM) and takes the next lane."""
 import numpy as np
 import pytest
 
@@ -35,8 +37,24 @@ def test_batched_config1(blocks, split):
     s = comp.flatten(blocks)
     seeds, probs, cycles = _lanes(70, 20, 120, 1)
     run = BatchedRun(s, seeds, probs, cycles)
-    print("groups", run.G, "cycles per group", run.gcycles.tolist(), flush=True)
+    print("groups", run.G, flush=True)
     _check(s, run, seeds, probs, cycles, range(70))
+    assert run.slot_cycles_total() >= run.lane_cycles_total()
+
+
+@pytest.mark.parametrize("groups", [1, 2])
+def test_batched_lane_queue(groups):
+    """70 lanes through 32 or 64 slots: most lanes start mid-run in a slot
+    another lane used before."""
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    s = comp.flatten(8)
+    seeds, probs, cycles = _lanes(70, 5, 90, 3)
+    run = BatchedRun(s, seeds, probs, cycles, groups=groups)
+    _check(s, run, seeds, probs, cycles, range(70))
+    gc = run.group_cycles()
+    print("group cycles", gc.tolist(), "lane cycles", run.lane_cycles_total(), flush=True)
+    # every slot stays busy until the queue runs dry
+    assert gc.max() <= -(-run.lane_cycles_total() // (32 * groups)) + int(cycles.max())
 
 
 def test_batched_srams_dups_major_stages():
@@ -46,3 +64,13 @@ def test_batched_srams_dups_major_stages():
     seeds, probs, cycles = _lanes(40, 10, 60, 2)
     run = BatchedRun(s, seeds, probs, cycles)
     _check(s, run, seeds, probs, cycles, [0, 1, 7, 19, 33, 39])
+
+
+def test_batched_srams_through_queue():
+    """SRAM lanes that reuse a slot start from zeroed SRAM contents."""
+    nl, comp = design("sram", level_split=8, seed=4, gates=30000, depth=48,
+                      dffs=3000, inputs=64, outputs=32, geometric_p=0.3, srams=4)
+    s = comp.flatten(16)
+    seeds, probs, cycles = _lanes(48, 10, 40, 4)
+    run = BatchedRun(s, seeds, probs, cycles, groups=1)
+    _check(s, run, seeds, probs, cycles, range(48))
