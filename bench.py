@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--batched-groups", type=int, default=0,
                     help="lane groups of 32 slots (0 = one per 32 lanes)")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="batched form with the heuristic block assignment only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--verify-cycles", type=int, default=4)
@@ -180,9 +182,13 @@ def bench_plain(a, script, torch):
 def bench_batched(a, comp, torch):
     """Batched form on the same netlist: --lanes lanes with per-lane seeds,
     cycle counts U[10K, 200K] x --batched-scale and toggle rates
-    U[0.02, 0.5], in one persistent launch with the device lane queue.
-    Groups and blocks per group come from the autotuner unless given."""
+    U[0.02, 0.5], in one persistent launch with the device lane queue,
+    captured with its reset as a hipGraph and timed as one graph replay.
+    Groups and blocks per group come from the autotuner unless given; the
+    block assignment uses the learned partition costs (trained here on the
+    kernel's own partition timings) unless --no-learned."""
     from jacquard_amd import batched as B
+    from jacquard_amd import costmodel as CM
     rng = np.random.default_rng(a.seed + 100)
     lo, hi = int(10_000 * a.batched_scale), int(200_000 * a.batched_scale)
     seeds = rng.integers(1, 2**31, a.lanes).astype(np.uint32)
@@ -192,20 +198,36 @@ def bench_batched(a, comp, torch):
     if a.batched_blocks:
         W = a.batched_blocks
         G = a.batched_groups or (a.lanes + 31) // 32
-        script = comp.flatten(W)
     else:
         t = time.time()
         tune = B.autotune(comp, cycles, log=log)
-        W, G, script = tune["blocks"], tune["groups"], tune["script"]
+        W, G = tune["blocks"], tune["groups"]
         log(f"autotune: chose {G} groups x {W} blocks in {time.time() - t:.1f}s")
+    learned = None
+    costs = None
+    if not a.no_learned:
+        costs, _model, info = CM.learn_costs(comp, W, G, log=log)
+        ab = CM.compare_assignments(comp, W, G, {"heuristic": None, "learned": costs},
+                                    repeats=3, cycles=32)
+        learned = dict(info, us_per_cycle=ab, gain=ab["heuristic"] / ab["learned"])
+        log(f"cost model: us/cycle heuristic {ab['heuristic']:.1f} learned "
+            f"{ab['learned']:.1f}")
+        if ab["learned"] > ab["heuristic"]:
+            costs = None  # the heuristic stays the fallback; measured time judges
+            learned["used"] = "heuristic"
+        else:
+            learned["used"] = "learned"
+    script = comp.flatten(W, costs=costs)
     run = B.BatchedRun(script, seeds, probs, cycles, groups=G)
     script.validate(script.reg_io_state_size)
     run.prepare()
     torch.cuda.synchronize()
+    run.capture_graph()
+    torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record()
-    run.launch()
+    run.graph.replay()
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -231,7 +253,7 @@ def bench_batched(a, comp, torch):
         checked += v
     return dict(wall=wall, kernel_ms=e0.elapsed_time(e1), lane_cycles=lc,
                 slot_cycles=run.slot_cycles_total(), groups=G, blocks=W,
-                verified_cycles=checked, script=script,
+                verified_cycles=checked, script=script, learned=learned,
                 tune=None if tune is None else tune["trials"])
 
 
@@ -477,6 +499,8 @@ def main():
                             "slot_utilisation": b["lane_cycles"] / b["slot_cycles"],
                             "kernel_s": b["kernel_ms"] / 1e3,
                             "autotune": b["tune"],
+                            "learned_cost_model": b["learned"],
+                            "launch": "one hipGraph replay (reset + persistent kernel)",
                             "verified_cycles": b["verified_cycles"]}
         forms["batched"]["vs_plain"] = forms["batched"]["lane_cycles_per_s"] / value
         log(f"batched: {forms['batched']['lane_cycles_per_s']:,.0f} lane-cycles/s "

@@ -132,6 +132,23 @@ int jq_simulate_batched(usize num_groups, usize blocks_per_group,
                         u32 *group_cycles, const u32 *obs_bits, u32 num_obs,
                         u32 obs_capacity, u32 *obs_out, void *stream);
 
+/* Per-partition timing samples of the batched form (no counterpart in the
+ * reference; the learned cost model trains on them).  Same launch as
+ * jq_simulate_batched; additionally, for the first sample_cycles cycles of
+ * group 0, writes the clock ticks of every partition execution into
+ * samples[cycle * num_parts + part] (part = order of appearance in
+ * blocks_data, as in gather_table). */
+int jq_simulate_batched_profiled(
+    usize num_groups, usize blocks_per_group, usize num_major_stages,
+    const usize *blocks_start, const u32 *blocks_data, const u32 *gather_table,
+    const u32 *gather_start, u32 reg_io_state_size, u32 sram_storage_size,
+    u32 *sram_data, u32 *states, const u32 *data_bits, u32 n_data,
+    const u32 *flag_bits, u32 n_flag, u32 n_lanes, const u32 *lane_cycles,
+    const u32 *lane_seed, const u32 *lane_thresh, u32 *lane_info, u32 *slots,
+    u32 *queue, u32 *group_cycles, const u32 *obs_bits, u32 num_obs,
+    u32 obs_capacity, u32 *obs_out, u32 num_parts, u32 *samples,
+    u32 sample_cycles, void *stream);
+
 /* Occupancy query used by the autotuner: the largest number of resident
  * workgroups (all co-resident, as the persistent launches require) for the
  * scan kernel (kind 0) or the batched kernel (kind 1) on the current device. */

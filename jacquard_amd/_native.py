@@ -98,6 +98,7 @@ HIP_SYMBOLS = [
     "jq_simulate_scan",
     "jq_simulate_scan_profiled",
     "jq_simulate_batched",
+    "jq_simulate_batched_profiled",
     "jq_max_resident_blocks",
     "jq_mlp_forward",
     "jq_mlp_train_step",
@@ -131,6 +132,11 @@ def hip():
             c_u64, c_u64, c_u64, c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp,
             c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp, c_vp, c_vp, c_vp,
             c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp, c_vp]),
+        "jq_simulate_batched_profiled": (ctypes.c_int, [
+            c_u64, c_u64, c_u64, c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp,
+            c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp, c_vp, c_vp, c_vp,
+            c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp, c_u32, c_vp, c_u32,
+            c_vp]),
         "jq_max_resident_blocks": (ctypes.c_int, [ctypes.c_int,
                                                   P(ctypes.c_int)]),
         "jq_mlp_forward": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int,

@@ -147,24 +147,65 @@ class BatchedRun:
             obs=torch.zeros(max(self.G * self.obs_cap * len(self.obs_bits), 1),
                             dtype=torch.int32, device="cuda"),
         )
+        # pristine copies of everything the launch updates in place
+        self.d0 = {k: self.d[k].clone() for k in ("states", "info", "slots", "queue")}
         return self.d
+
+    def reset(self):
+        """Restore the launch's in/out buffers to their initial contents
+        with device copies and fill kernels (no memset nodes, so the
+        sequence can be captured in a graph)."""
+        for k, v in self.d0.items():
+            self.d[k].copy_(v)
+        self.d["sram"].zero_()
+        self.d["gcyc"].zero_()
+        self.d["obs"].zero_()
+
+    def capture_graph(self):
+        """A hipGraph of the whole batched run (reset + the persistent
+        launch) on a side stream; replay() reruns every lane."""
+        import torch
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.reset()
+                self.launch(stream=s)
+        torch.cuda.current_stream().wait_stream(s)
+        self.graph = g
+        return g
+
+    def _args(self):
+        s = self.script
+        d = self.d
+        return [self.G, s.num_blocks, s.num_major_stages, d["bs"].data_ptr(),
+                d["bd"].data_ptr(), d["gtab"].data_ptr(), d["gstart"].data_ptr(),
+                s.reg_io_state_size, s.sram_storage_size, d["sram"].data_ptr(),
+                d["states"].data_ptr(), d["data_bits"].data_ptr(), len(self.data_bits),
+                d["flag_bits"].data_ptr(), len(self.flag_bits), len(self.seeds),
+                d["lcyc"].data_ptr(), d["seed"].data_ptr(), d["thr"].data_ptr(),
+                d["info"].data_ptr(), d["slots"].data_ptr(), d["queue"].data_ptr(),
+                d["gcyc"].data_ptr(), d["obs_bits"].data_ptr(), len(self.obs_bits),
+                self.obs_cap, d["obs"].data_ptr()]
 
     def launch(self, stream=None):
         import torch
-        s = self.script
-        d = self.d
         st = stream if stream is not None else torch.cuda.current_stream()
-        rc = N.hip().jq_simulate_batched(
-            self.G, s.num_blocks, s.num_major_stages, d["bs"].data_ptr(),
-            d["bd"].data_ptr(), d["gtab"].data_ptr(), d["gstart"].data_ptr(),
-            s.reg_io_state_size, s.sram_storage_size, d["sram"].data_ptr(),
-            d["states"].data_ptr(), d["data_bits"].data_ptr(), len(self.data_bits),
-            d["flag_bits"].data_ptr(), len(self.flag_bits), len(self.seeds),
-            d["lcyc"].data_ptr(), d["seed"].data_ptr(), d["thr"].data_ptr(),
-            d["info"].data_ptr(), d["slots"].data_ptr(), d["queue"].data_ptr(),
-            d["gcyc"].data_ptr(), d["obs_bits"].data_ptr(), len(self.obs_bits),
-            self.obs_cap, d["obs"].data_ptr(), ctypes.c_void_p(st.cuda_stream))
-        N.check(rc)
+        N.check(N.hip().jq_simulate_batched(*self._args(), ctypes.c_void_p(st.cuda_stream)))
+
+    def launch_profiled(self, sample_cycles, stream=None):
+        """Launch with per-partition clock samples of group 0 for the first
+        sample_cycles cycles; returns the device sample buffer
+        (uint32[sample_cycles, num_parts] once synchronised)."""
+        import torch
+        n = self.script.num_parts
+        samples = torch.zeros(max(sample_cycles * n, 1), dtype=torch.int32, device="cuda")
+        st = stream if stream is not None else torch.cuda.current_stream()
+        N.check(N.hip().jq_simulate_batched_profiled(
+            *self._args(), n, samples.data_ptr(), sample_cycles,
+            ctypes.c_void_p(st.cuda_stream)))
+        return samples
 
     def run(self, validate=True):
         import torch

@@ -3,13 +3,16 @@
 A small FP32 MLP (24 script features -> 64 -> 64 -> 1) predicts the GPU
 latency of one partition execution from its script section
 (``FlattenedScriptV1.part_features``).  Training data comes from the
-simulator itself: the profiled interpreter (jq_simulate_scan_profiled)
-records the clock ticks of every partition execution.  Training and
-inference run on the GPU through the hand-written kernels in
-libjacquard_hip.so (jq_mlp_*).  The predictions replace the reference's
-heuristic cost (boomerang stages + 3, src/flatten.rs:1207-1226) in the
-masonry block assignment; the heuristic stays the fallback, and measured
-kernel time is the judge.
+simulator itself: the profiled launches (jq_simulate_scan_profiled,
+jq_simulate_batched_profiled) record the clock ticks of every partition
+execution.  Training and inference run on the GPU through the hand-written
+kernels in libjacquard_hip.so (jq_mlp_*).  The predictions replace the
+reference's heuristic cost (boomerang stages + 3, src/flatten.rs:1207-1226)
+in the block assignment (longest-predicted-first onto the least-loaded
+block); the heuristic stays the fallback, and measured kernel time is the
+judge (``compare_assignments``).  Features depend only on a partition's own
+script section, not on which block it lands in, so costs predicted from one
+flattening apply to any other.
 """
 import ctypes
 
@@ -152,3 +155,124 @@ def predicted_makespan(script, per_part):
         np.add.at(loads, script.parts[sel, 1].astype(np.int64), per_part[sel])
         tot += loads.max() if loads.size else 0.0
     return tot
+
+
+# ----------------------------------------------------------------------
+# batched form: samples, training set, assignment A/B
+
+def profile_batched(compiled, blocks, groups, sample_cycles, costs=None, seed=0):
+    """Flatten for `blocks` blocks per group, run `groups` full groups of
+    lanes for sample_cycles + 2 cycles with per-partition clock sampling on
+    group 0.  Returns (script, ticks float64[sample_cycles, num_parts]) in
+    blocks_data partition order."""
+    torch = _torch()
+    from .batched import BatchedRun
+    s = compiled.flatten(blocks, costs=costs)
+    n = 32 * groups
+    rng = np.random.default_rng(seed)
+    run = BatchedRun(s, rng.integers(1, 2**31, n), rng.uniform(0.02, 0.5, n),
+                     np.full(n, sample_cycles + 2), obs_lits=[], groups=groups)
+    run.prepare()
+    torch.cuda.synchronize()
+    samp = run.launch_profiled(sample_cycles)
+    torch.cuda.synchronize()
+    ticks = samp.cpu().numpy().view(np.uint32)[:sample_cycles * s.num_parts]
+    return s, ticks.reshape(sample_cycles, s.num_parts).astype(np.float64)
+
+
+def training_set(compiled, blocks_list, groups, sample_cycles, skip=2, seed=0):
+    """(features float32[n, 24], ticks float64[n], per-partition median
+    ticks by (major, index)) from profiled batched launches of several
+    flattenings; every (cycle, partition) execution after the first `skip`
+    cycles is one sample."""
+    xs, ys = [], []
+    med = {}
+    for k, W in enumerate(blocks_list):
+        s, ticks = profile_batched(compiled, W, groups, sample_cycles + skip, seed=seed + k)
+        f = s.part_features()
+        t = ticks[skip:]
+        xs.append(np.repeat(f[None], t.shape[0], 0).reshape(-1, f.shape[1]))
+        ys.append(t.reshape(-1))
+        m = np.median(t, 0)
+        for (mj, _b, idx, _o, _l), v in zip(s.parts.tolist(), m.tolist()):
+            med.setdefault((mj, idx), []).append(v)
+    med = {k: float(np.median(v)) for k, v in med.items()}
+    return np.concatenate(xs).astype(np.float32), np.concatenate(ys), med
+
+
+def predicted_costs(compiled, model, blocks):
+    """Per-partition costs [major][index] predicted by `model` from the
+    features of a flattening (any block count)."""
+    s = compiled.flatten(blocks)
+    y = model.predict(s.part_features())
+    return stage_costs(compiled, s, y), s
+
+
+def measured_costs(compiled, med):
+    costs = [np.zeros(n) for n in compiled.parts_per_stage()]
+    for (mj, idx), v in med.items():
+        costs[mj][idx] = v
+    return costs
+
+
+def batched_us_per_cycle(script, groups, cycles=48, seed=1):
+    """Kernel time per group cycle of the batched launch of `script` with
+    `groups` full groups of lanes."""
+    torch = _torch()
+    from .batched import BatchedRun
+    n = 32 * groups
+    rng = np.random.default_rng(seed)
+    run = BatchedRun(script, rng.integers(1, 2**31, n), rng.uniform(0.02, 0.5, n),
+                     np.full(n, cycles), obs_lits=[], groups=groups)
+    run.prepare()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    run.launch()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e3 / cycles
+
+
+def compare_assignments(compiled, blocks, groups, variants, repeats=3, cycles=48):
+    """Measured us/cycle of the batched kernel for each named cost variant
+    ({name: costs or None}); the variants' launches are interleaved
+    `repeats` times and the median is returned per name."""
+    scripts = {k: compiled.flatten(blocks, costs=c) for k, c in variants.items()}
+    out = {k: [] for k in variants}
+    for r in range(repeats):
+        for k in variants:
+            out[k].append(batched_us_per_cycle(scripts[k], groups, cycles, seed=r + 1))
+    return {k: float(np.median(v)) for k, v in out.items()}
+
+
+def learn_costs(compiled, blocks, groups, sample_cycles=24, epochs=150, log=None):
+    """Train the MLP on profiled batched launches of the script flattened
+    for `blocks`, `blocks`-8 and `blocks`-16 blocks per group (distinct
+    partition-to-block placements), and return (costs [major][index],
+    model, info)."""
+    import time
+    widths = sorted({blocks, max(4, blocks - 8), max(4, blocks - 16)}, reverse=True)
+    t = time.time()
+    X, y, med = training_set(compiled, widths, groups, sample_cycles)
+    t_prof = time.time() - t
+    t = time.time()
+    model = MLP(seed=1)
+    hist = model.fit(X, y, epochs=epochs, lr=0.05, batch=4096)
+    t_fit = time.time() - t
+    s0 = compiled.flatten(blocks)
+    f0 = s0.part_features()
+    pred = model.predict(f0)
+    truth = np.array([med[(m, i)] for (m, _b, i, _o, _l) in s0.parts.tolist()])
+    h = f0[:, 0] + 3.0
+    a, b = np.polyfit(h, truth, 1)
+    info = dict(samples=int(len(y)), profile_s=t_prof, fit_s=t_fit,
+                loss_first=float(hist[0]), loss_last=float(hist[-1]),
+                mape_heuristic_fit=float(np.mean(np.abs(a * h + b - truth) / truth)),
+                mape_learned=float(np.mean(np.abs(pred - truth) / truth)))
+    if log:
+        log(f"cost model: {info['samples']} samples, loss {info['loss_first']:.3f} -> "
+            f"{info['loss_last']:.3f}, MAPE heuristic-fit {info['mape_heuristic_fit']:.3f} "
+            f"learned {info['mape_learned']:.3f}")
+    return stage_costs(compiled, s0, pred), model, info

@@ -71,6 +71,9 @@ struct BArgs {
   u32 n_obs, obs_cap;
   u32 *obs_out;
   u32 *bar;
+  // optional per-partition clock samples of group 0: prof[c*n_parts + part]
+  u32 *prof;
+  u32 prof_cycles, n_parts;
 };
 
 constexpr u32 kNoLane = 0xFFFFFFFFu;
@@ -133,7 +136,7 @@ __device__ __forceinline__ void sec_wait() { __builtin_amdgcn_s_waitcnt(0x0F70);
 __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__restrict__ script,
                                 const u32 *__restrict__ gsrc,
                                 const u32 *__restrict__ in, u32 *__restrict__ out,
-                                u32 *ram_group, u32 *lds) {
+                                u32 *ram_group, u32 *lds, u32 *prof) {
   const u32 t = threadIdx.x;
   const u32 lane = t & 63;
   u32 *bufA = lds, *bufB = lds + kLayerWords, *WO = lds + 2 * kLayerWords;
@@ -143,6 +146,7 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
   for (;; gsrc += kLayerWords) {
     const u32 S = script[pi];
     if (S == 0) break;
+    const unsigned long long tick0 = prof ? clock64() : 0;
     const u32 is_last = script[pi + 1];
     const u32 num_ios = script[pi + 2];
     const u32 io = script[pi + 3];
@@ -311,6 +315,10 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
       }
     }
     __syncthreads();
+    if (prof) {
+      if (t == 0) *prof = (u32)(clock64() - tick0);
+      ++prof;
+    }
     if (is_last) break;
   }
 }
@@ -404,8 +412,10 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
     }
     for (u32 mj = 0; mj < A.num_major; ++mj) {
       const size_t k = (size_t)mj * A.W + b;
+      u32 *prof = (A.prof && g == 0 && c < A.prof_cycles)
+                      ? A.prof + (size_t)c * A.n_parts + A.gidx[k] : nullptr;
       run_block_lanes(A, A.bd + A.bs[k], A.gsrc + (size_t)A.gidx[k] * kLayerWords, in,
-                      out, ram_group, lds);
+                      out, ram_group, lds, prof);
       grid_sync(bar, A.W, gen, &gb_error);
     }
   }
@@ -449,15 +459,18 @@ int batched_blocks_per_cu(int *out) {
 
 using namespace jqd;
 
-extern "C" int jq_simulate_batched(
-    usize num_groups, usize blocks_per_group, usize num_major_stages,
-    const usize *blocks_start, const u32 *blocks_data, const u32 *gather_table,
-    const u32 *gather_start, u32 reg_io_state_size, u32 sram_storage_size,
-    u32 *sram_data, u32 *states, const u32 *data_bits, u32 n_data,
-    const u32 *flag_bits, u32 n_flag, u32 n_lanes, const u32 *lane_cycles,
-    const u32 *lane_seed, const u32 *lane_thresh, u32 *lane_info, u32 *slots,
-    u32 *queue, u32 *group_cycles, const u32 *obs_bits, u32 num_obs,
-    u32 obs_capacity, u32 *obs_out, void *stream) {
+namespace {
+
+int batched_impl(usize num_groups, usize blocks_per_group, usize num_major_stages,
+                 const usize *blocks_start, const u32 *blocks_data,
+                 const u32 *gather_table, const u32 *gather_start,
+                 u32 reg_io_state_size, u32 sram_storage_size, u32 *sram_data,
+                 u32 *states, const u32 *data_bits, u32 n_data, const u32 *flag_bits,
+                 u32 n_flag, u32 n_lanes, const u32 *lane_cycles, const u32 *lane_seed,
+                 const u32 *lane_thresh, u32 *lane_info, u32 *slots, u32 *queue,
+                 u32 *group_cycles, const u32 *obs_bits, u32 num_obs,
+                 u32 obs_capacity, u32 *obs_out, u32 num_parts, u32 *samples,
+                 u32 sample_cycles, void *stream) {
   set_error("");
   if (num_groups == 0 || blocks_per_group == 0) return 0;
   if (num_groups > (usize)kBarrierSlots) {
@@ -467,9 +480,13 @@ extern "C" int jq_simulate_batched(
       !group_cycles) {
     return set_error("batched launch: missing gather table or lane-queue buffers");
   }
-  int per = 0;
-  if (batched_blocks_per_cu(&per)) {
+  int per = 0, dev = 0, cus = 0;
+  if (batched_blocks_per_cu(&per) || hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
     return set_error("batched occupancy query failed");
+  }
+  if (num_groups * blocks_per_group > (usize)per * (usize)cus) {
+    return set_error("batched launch: groups x blocks exceed the co-resident workgroups");
   }
   BArgs A{};
   A.G = (u32)num_groups;
@@ -499,6 +516,9 @@ extern "C" int jq_simulate_batched(
   A.n_obs = num_obs;
   A.obs_cap = obs_capacity;
   A.obs_out = obs_out;
+  A.prof = samples;
+  A.prof_cycles = samples ? sample_cycles : 0;
+  A.n_parts = num_parts;
   u32 *bar_base = nullptr;
   if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(gb_barrier)) != hipSuccess) {
     return set_error("hipGetSymbolAddress(gb_barrier) failed");
@@ -507,12 +527,64 @@ extern "C" int jq_simulate_batched(
   if (first + num_groups > (usize)kBarrierSlots) first = 0;
   A.bar = bar_base + kSlotWords * first;
   void *args[] = {&A};
-  hipError_t e = hipLaunchCooperativeKernel(
-      reinterpret_cast<const void *>(&batched_kernel),
-      dim3((u32)(num_groups * blocks_per_group)), dim3(256), args, kLdsWords * 4,
-      (hipStream_t)stream);
+  // Under stream capture (a hipGraph of the run) the launch is an ordinary
+  // kernel node: co-residency of all workgroups is established above from
+  // the occupancy query, and the group barriers' bounded wait turns any
+  // shortfall into an error flag instead of a hang.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing((hipStream_t)stream, &cap) != hipSuccess) {
+    return set_error("hipStreamIsCapturing failed");
+  }
+  hipError_t e;
+  if (cap == hipStreamCaptureStatusActive) {
+    e = hipLaunchKernel(reinterpret_cast<const void *>(&batched_kernel),
+                        dim3((u32)(num_groups * blocks_per_group)), dim3(256), args,
+                        kLdsWords * 4, (hipStream_t)stream);
+  } else {
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&batched_kernel),
+                                   dim3((u32)(num_groups * blocks_per_group)), dim3(256),
+                                   args, kLdsWords * 4, (hipStream_t)stream);
+  }
   if (e != hipSuccess) {
     return set_error((std::string("batched launch failed: ") + hipGetErrorString(e)).c_str());
   }
   return 0;
+}
+
+}  // namespace
+
+extern "C" int jq_simulate_batched(
+    usize num_groups, usize blocks_per_group, usize num_major_stages,
+    const usize *blocks_start, const u32 *blocks_data, const u32 *gather_table,
+    const u32 *gather_start, u32 reg_io_state_size, u32 sram_storage_size,
+    u32 *sram_data, u32 *states, const u32 *data_bits, u32 n_data,
+    const u32 *flag_bits, u32 n_flag, u32 n_lanes, const u32 *lane_cycles,
+    const u32 *lane_seed, const u32 *lane_thresh, u32 *lane_info, u32 *slots,
+    u32 *queue, u32 *group_cycles, const u32 *obs_bits, u32 num_obs,
+    u32 obs_capacity, u32 *obs_out, void *stream) {
+  return batched_impl(num_groups, blocks_per_group, num_major_stages, blocks_start,
+                      blocks_data, gather_table, gather_start, reg_io_state_size,
+                      sram_storage_size, sram_data, states, data_bits, n_data, flag_bits,
+                      n_flag, n_lanes, lane_cycles, lane_seed, lane_thresh, lane_info,
+                      slots, queue, group_cycles, obs_bits, num_obs, obs_capacity, obs_out,
+                      0, nullptr, 0, stream);
+}
+
+extern "C" int jq_simulate_batched_profiled(
+    usize num_groups, usize blocks_per_group, usize num_major_stages,
+    const usize *blocks_start, const u32 *blocks_data, const u32 *gather_table,
+    const u32 *gather_start, u32 reg_io_state_size, u32 sram_storage_size,
+    u32 *sram_data, u32 *states, const u32 *data_bits, u32 n_data,
+    const u32 *flag_bits, u32 n_flag, u32 n_lanes, const u32 *lane_cycles,
+    const u32 *lane_seed, const u32 *lane_thresh, u32 *lane_info, u32 *slots,
+    u32 *queue, u32 *group_cycles, const u32 *obs_bits, u32 num_obs,
+    u32 obs_capacity, u32 *obs_out, u32 num_parts, u32 *samples, u32 sample_cycles,
+    void *stream) {
+  if (!samples || num_parts == 0) return set_error("profiled batched launch needs samples");
+  return batched_impl(num_groups, blocks_per_group, num_major_stages, blocks_start,
+                      blocks_data, gather_table, gather_start, reg_io_state_size,
+                      sram_storage_size, sram_data, states, data_bits, n_data, flag_bits,
+                      n_flag, n_lanes, lane_cycles, lane_seed, lane_thresh, lane_info,
+                      slots, queue, group_cycles, obs_bits, num_obs, obs_capacity, obs_out,
+                      num_parts, samples, sample_cycles, stream);
 }

@@ -74,3 +74,24 @@ def test_batched_srams_through_queue():
     seeds, probs, cycles = _lanes(48, 10, 40, 4)
     run = BatchedRun(s, seeds, probs, cycles, groups=1)
     _check(s, run, seeds, probs, cycles, range(48))
+
+
+def test_batched_graph_replays():
+    """The reset + launch sequence captured as one hipGraph: three replays,
+    with eager work and synchronisation between them, each reproduce the
+    eager run's outputs for every lane."""
+    import torch
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    s = comp.flatten(8)
+    seeds, probs, cycles = _lanes(50, 5, 60, 7)
+    run = BatchedRun(s, seeds, probs, cycles, groups=1)
+    eager = run.run()
+    run.capture_graph()
+    for r in range(3):
+        run.graph.replay()
+        torch.cuda.synchronize()
+        scratch = torch.ones(1 << 20, device="cuda").sum().item()  # eager work between
+        assert scratch == float(1 << 20)
+        got = run.lane_outputs()
+        for lane in range(50):
+            assert np.array_equal(got[lane], eager[lane]), (r, lane)

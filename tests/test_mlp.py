@@ -77,3 +77,33 @@ def test_gpu_training_reduces_loss():
     m = MLP(seed=1)
     hist = m.fit(x, np.exp(t), epochs=60, lr=0.05, batch=512)
     assert hist[-1] < 0.5 * hist[0]
+
+
+@pytest.mark.gpu
+def test_learned_costs_against_heuristic():
+    """The model trained on the batched kernel's own partition timings
+    predicts held-out partition latency better than the reference heuristic
+    (boomerang stages + 3, best linear fit), and the block assignment built
+    from its predictions does not run slower than the heuristic's."""
+    from helpers import design
+    from jacquard_amd import costmodel as CM
+    nl, comp = design("cm", level_split=4, seed=2, gates=300000, depth=48, dffs=20000,
+                      inputs=256, outputs=128, geometric_p=0.3, invert_prob=0.5)
+    W, G = 12, 2
+    X, y, med = CM.training_set(comp, [W, 10, 8], G, 16)
+    assert len(y) >= 5000
+    s0 = comp.flatten(W)
+    f0 = s0.part_features()
+    truth = np.array([med[(m, i)] for (m, _b, i, _o, _l) in s0.parts.tolist()])
+    h = f0[:, 0] + 3
+    a, b = np.polyfit(h, truth, 1)
+    mape_h = float(np.mean(np.abs(a * h + b - truth) / truth))
+    model = CM.MLP(seed=1)
+    model.fit(X, y, epochs=150, lr=0.05, batch=4096)
+    mape_l = float(np.mean(np.abs(model.predict(f0) - truth) / truth))
+    r = CM.compare_assignments(comp, W, G, {
+        "reference": None, "learned": CM.stage_costs(comp, s0, model.predict(f0))},
+        repeats=3, cycles=32)
+    print(f"MAPE heuristic {mape_h:.3f} learned {mape_l:.3f}; us/cycle {r}", flush=True)
+    assert mape_l < mape_h
+    assert r["learned"] <= 1.03 * r["reference"]
