@@ -43,10 +43,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cycles-per-step", type=int, default=10_000)
-    ap.add_argument("--workload", choices=["plain", "xtiming"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "xtiming", "lanes4m"], default="plain",
                     help="plain: the 1M-gate 2-state drop-in (the headline line); "
                          "xtiming: the 4M-gate netlist with 8 SRAMs, 10%% initial-X "
-                         "DFFs, 20-200 ps gate delays and 50/20 ps setup/hold at 5 ns")
+                         "DFFs, 20-200 ps gate delays and 50/20 ps setup/hold at 5 ns; "
+                         "lanes4m: batched+learned lanes vs the plain drop-in on that "
+                         "4M-gate netlist (value lanes)")
     ap.add_argument("--gates", type=int, default=None)
     ap.add_argument("--depth", type=int, default=None)
     ap.add_argument("--dffs", type=int, default=None)
@@ -79,7 +81,7 @@ def parse():
                     help="first run an elementwise copy of MIB MiB three times "
                          "(the known-size kernel that scales the HBM counters)")
     a = ap.parse_args()
-    big = a.workload == "xtiming"
+    big = a.workload in ("xtiming", "lanes4m")
     if a.gates is None:
         a.gates = 4_000_000 if big else 1_000_000
     if a.depth is None:
@@ -217,7 +219,7 @@ def bench_batched(a, comp, torch):
             learned["used"] = "heuristic"
         else:
             learned["used"] = "learned"
-    script = comp.flatten(W, costs=costs)
+    script = comp.flatten_shared(W) if costs is None else comp.flatten(W, costs=costs)
     run = B.BatchedRun(script, seeds, probs, cycles, groups=G)
     script.validate(script.reg_io_state_size)
     run.prepare()
@@ -391,6 +393,65 @@ def bench_xtiming(a, torch):
                 cpu_s=cpu_s, verified_cycles=2 * V, counts=nl.counts())
 
 
+def bench_lanes4m(a, torch):
+    """Batched+learned lanes against the plain drop-in on the 4M-gate netlist
+    of the X/timing workload (same generator: 8 single-port 1024x32 SRAMs,
+    per-gate delays), both as 2-state value runs: the plain kernel runs
+    --steps x --cycles-per-step cycles after --warmup steps; the batched form
+    runs --lanes lanes of U[10K, 200K] x --batched-scale cycles."""
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    import ctypes
+    from jacquard_amd import _native as N
+    t = time.time()
+    nl = Netlist.random(seed=a.seed, gates=a.gates, depth=a.depth, dffs=a.dffs,
+                        inputs=a.inputs, outputs=a.outputs, geometric_p=0.3,
+                        invert_prob=0.5, srams=8, sram_addr_bits=10, delay=(20, 200))
+    comp = Compiled(nl, level_split=a.split)
+    log(f"lanes4m: netlist {nl.counts()} compiled in {time.time() - t:.1f}s")
+    res = ctypes.c_int(0)
+    N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
+    script = comp.flatten(min(max(comp.parts_per_stage()), res.value))
+    p = bench_plain(a, script, torch)
+    plain = p["cycles"] / p["wall"]
+    log(f"lanes4m: plain {plain:,.0f} cycles/s")
+    b = bench_batched(a, comp, torch)
+    value = b["lane_cycles"] / b["wall"]
+    bs = b["script"]
+    bpc = bs.algorithmic_bytes_per_cycle()
+    # a group cycle moves the script once for its 32 lanes, plus 32 lanes'
+    # worth of state traffic: bytes per lane-cycle = W/32 + state part
+    per_lane = (4 * int(bs.blocks_start[-1])) / 32 + (bpc - 4 * int(bs.blocks_start[-1]))
+    achieved = per_lane * b["lane_cycles"] / (b["kernel_ms"] / 1e3) / 1e9
+    print(json.dumps({
+        "metric": METRIC, "value": value, "unit": "lane-cycles/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": b["wall"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic",
+        "config": {
+            "workload": f"batched+learned lanes vs plain drop-in: seeded random AIG, "
+                        f"{a.gates} AND, depth {a.depth}, {a.dffs} DFF, 8 x 1024x32 SRAM; "
+                        f"{a.lanes} lanes, cycles U[{int(10_000 * a.batched_scale)}, "
+                        f"{int(200_000 * a.batched_scale)}], toggle U[0.02, 0.5]",
+            "batched": {"lanes": a.lanes, "groups": b["groups"],
+                        "blocks_per_group": b["blocks"], "lane_cycles": b["lane_cycles"],
+                        "slot_utilisation": b["lane_cycles"] / b["slot_cycles"],
+                        "learned_cost_model": b["learned"], "autotune": b["tune"],
+                        "launch": "one hipGraph replay (reset + persistent kernel)",
+                        "verified_cycles": b["verified_cycles"]},
+            "plain": {"lane_cycles_per_s": plain, "blocks": script.num_blocks,
+                      "verified_cycles": p.get("verified_cycles")},
+            "vs_plain": value / plain,
+            "major_stages": script.num_major_stages, "partitions": script.num_parts,
+            "script_bytes": int(script.blocks_data.nbytes)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "batched_kernel",
+                     "algorithmic_bytes_per_lane_cycle": per_lane},
+        "cpu_baseline": None,
+    }), flush=True)
+
+
 def cpu_baseline(a, script):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -430,7 +491,8 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(lanes would shard across GPUs, see DESIGN.md)"}))
+                          "(lanes would shard across GPUs with one script replica "
+                          "each; only the one-GPU share is built and measured)"}))
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
@@ -438,6 +500,8 @@ def main():
         sys.exit(1)
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
+    if a.workload == "lanes4m":
+        return bench_lanes4m(a, torch)
     if a.workload == "xtiming":
         x = bench_xtiming(a, torch)
         s = x["script"]

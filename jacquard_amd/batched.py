@@ -288,7 +288,7 @@ def autotune(compiled, lane_cycles, widths=(16, 24, 32, 48, 64), trial_cycles=24
     for W in widths:
         if W > resident:
             continue
-        s = compiled.flatten(W)
+        s = compiled.flatten_shared(W)
         gs = sorted({max(1, min(g_full, resident // W, g_full >> k)) for k in range(4)},
                     reverse=True)
         for G in gs:

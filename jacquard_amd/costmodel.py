@@ -167,7 +167,7 @@ def profile_batched(compiled, blocks, groups, sample_cycles, costs=None, seed=0)
     blocks_data partition order."""
     torch = _torch()
     from .batched import BatchedRun
-    s = compiled.flatten(blocks, costs=costs)
+    s = compiled.flatten_shared(blocks) if costs is None else compiled.flatten(blocks, costs=costs)
     n = 32 * groups
     rng = np.random.default_rng(seed)
     run = BatchedRun(s, rng.integers(1, 2**31, n), rng.uniform(0.02, 0.5, n),
@@ -239,7 +239,8 @@ def compare_assignments(compiled, blocks, groups, variants, repeats=3, cycles=48
     """Measured us/cycle of the batched kernel for each named cost variant
     ({name: costs or None}); the variants' launches are interleaved
     `repeats` times and the median is returned per name."""
-    scripts = {k: compiled.flatten(blocks, costs=c) for k, c in variants.items()}
+    scripts = {k: (compiled.flatten_shared(blocks) if c is None
+                   else compiled.flatten(blocks, costs=c)) for k, c in variants.items()}
     out = {k: [] for k in variants}
     for r in range(repeats):
         for k in variants:
@@ -261,7 +262,7 @@ def learn_costs(compiled, blocks, groups, sample_cycles=24, epochs=150, log=None
     model = MLP(seed=1)
     hist = model.fit(X, y, epochs=epochs, lr=0.05, batch=4096)
     t_fit = time.time() - t
-    s0 = compiled.flatten(blocks)
+    s0 = compiled.flatten_shared(blocks)
     f0 = s0.part_features()
     pred = model.predict(f0)
     truth = np.array([med[(m, i)] for (m, _b, i, _o, _l) in s0.parts.tolist()])

@@ -26,24 +26,23 @@ int set_error(const char *msg);
 // Host side: reads (and optionally clears) the batched kernels' error word.
 int batched_error_flag(u32 *out, int clear);
 
-// Grid-wide barrier among `n` co-resident workgroups.  Thread 0 of each
-// workgroup arrives with a device-scope release and waits with a
-// device-scope acquire (these also write back / invalidate the per-XCD L2
-// state that other workgroups read).  The wait is bounded: after ~2^27
-// polls it raises g_error and returns so the grid always drains.
-// A slot is kSlotWords u32 (256 bytes): count at [0], generation at [32],
-// so concurrent barriers never share a cache line.
+// Grid-wide barrier among `n` co-resident workgroups.  A slot is kSlotWords
+// u32 (256 bytes): arrival count at [0], generation at [32], on separate
+// 128-byte lines, so waiters polling the generation do not contend with the
+// arrivals' atomics, and concurrent barriers never share a line.  One
+// agent-scope release fence publishes the workgroup's stores (ordered after
+// them by the workgroup barrier), the arrival and the polling are relaxed
+// agent-scope atomics (an acquire on every poll would invalidate the XCD's
+// L2 once per poll, for every waiting workgroup), and one acquire fence
+// after the wait makes the other workgroups' stores visible.  The wait is
+// bounded: after ~2^27 polls it raises `err` and returns so the grid always
+// drains.  (Polling the arrival counter itself instead of a generation word
+// measured slower: the polls contend with the atomics on one line.)
 constexpr int kSlotWords = 64;
 
 __device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen, u32 *err) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    // One agent-scope release fence publishes this workgroup's stores
-    // (ordered after them by the barrier above), and one acquire fence after
-    // the wait makes the other workgroups' stores visible.  The arrival and
-    // the polling are relaxed agent-scope atomics: on gfx950 an acquire on
-    // every poll would invalidate the XCD's L2 once per poll, for every
-    // waiting workgroup.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     u32 arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);

@@ -25,7 +25,10 @@ class Netlist:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            N.host().jqh_aig_free(h)
+            try:
+                N.host().jqh_aig_free(h)
+            except Exception:  # interpreter shutdown: the library is gone
+                pass
             self._h = None
 
     # ------------------------------------------------------------ builders

@@ -36,7 +36,10 @@ class Compiled:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            N.host().jqh_compiled_free(h)
+            try:
+                N.host().jqh_compiled_free(h)
+            except Exception:  # interpreter shutdown: the library is gone
+                pass
             self._h = None
 
     @property
@@ -60,6 +63,14 @@ class Compiled:
 
     def flatten(self, num_blocks, xprop=False, costs=None):
         return FlattenedScriptV1(self, num_blocks, xprop=xprop, costs=costs)
+
+    def flatten_shared(self, num_blocks):
+        """2-state flattening with the heuristic cost, built once per block
+        count and shared; callers must not modify it (no timing injection)."""
+        cache = self.__dict__.setdefault("_shared", {})
+        if num_blocks not in cache:
+            cache[num_blocks] = self.flatten(num_blocks)
+        return cache[num_blocks]
 
 
 class FlattenedScriptV1:
@@ -132,7 +143,10 @@ class FlattenedScriptV1:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            N.host().jqh_script_free(h)
+            try:
+                N.host().jqh_script_free(h)
+            except Exception:  # interpreter shutdown: the library is gone
+                pass
             self._h = None
 
     # ------------------------------------------------------------------
