@@ -92,6 +92,18 @@ int jq_simulate_scan_profiled(usize num_blocks, usize num_major_stages,
  * of uneven length keep every slot busy.
  *   blocks_per_group : the script's num_blocks; the launch uses
  *                      num_groups * blocks_per_group co-resident workgroups
+ *   blocks_data      : the script with every boomerang-stage section in the
+ *                      lane layout (same size and offsets): words
+ *                      1024*(w>>2) + 4t + (w&3), w < 16, hold thread t's
+ *                      operand LDS indices m = 2w, 2w+1 (u16 each: the
+ *                      row-swizzled index p ^ ((p>>5)&31) of the previous
+ *                      layer position p = perm[t + 256m]); words 4096+4t+c,
+ *                      c < 3, hold the xora/xorb/orb bit of thread t's 31
+ *                      subtree gates (level-1 position 4096+t+256g for
+ *                      g < 16, then levels 2..5); word 4096+4t+3 (t < 64)
+ *                      the flags of positions 128+t, 192+t, 64+t, t at bits
+ *                      3i..3i+2.  Headers, global-read rounds, SRAM/dup and
+ *                      clock-enable sections are unchanged.
  *   gather_table     : u32[num_parts * 8192], the script's global reads
  *                      resolved per partition (blocks_data order): entry q
  *                      is the state bit landing at first-layer position q

@@ -83,6 +83,8 @@ def host():
                                         c_i32]),
         "jqh_part_features": (ctypes.c_int, [c_vp, c_u64, c_vp, c_u64,
                                              c_vp]),
+        "jqh_lane_blocks_data": (ctypes.c_int, [c_vp, c_u64, c_vp, c_u64,
+                                                c_vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

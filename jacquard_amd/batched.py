@@ -69,6 +69,45 @@ def gather_table(script):
     return table.reshape(-1), start
 
 
+def _lane_gate_positions():
+    """Positions of thread t's 31 subtree gates, gate index g: level 1
+    (g < 16) 4096 + t + 256g, level 2 2048 + t + 256(g-16), level 3
+    1024 + ..., level 4 512 + ..., level 5 256 + t.  [31, 256]"""
+    t = np.arange(256)
+    rows = []
+    for base, cnt in ((4096, 16), (2048, 8), (1024, 4), (512, 2), (256, 1)):
+        for k in range(cnt):
+            rows.append(base + t + 256 * k)
+    return np.stack(rows)
+
+
+
+def lane_blocks_data(script):
+    """blocks_data with every boomerang-stage section in the batched
+    kernel's lane layout (jqh_lane_blocks_data in libjacquard_host.so; the
+    layout is documented with jq_simulate_batched in include/jacquard_hip.h).
+    Headers, global-read rounds and the SRAM/dup and clock-enable sections
+    are unchanged, so blocks_start applies as is."""
+    bd = np.ascontiguousarray(script.blocks_data, np.uint32)
+    parts = np.ascontiguousarray(script.parts, np.uint64)
+    out = np.empty_like(bd)
+    if N.host().jqh_lane_blocks_data(bd.ctypes.data, bd.size, parts.ctypes.data,
+                                     len(parts), out.ctypes.data) != 0:
+        raise RuntimeError(N.host_error())
+    return out
+
+
+def batched_program(script):
+    """(lane blocks_data, gather table, gather start) of a script, built
+    once and kept on the script object."""
+    prog = getattr(script, "_batched_program", None)
+    if prog is None:
+        table, start = gather_table(script)
+        prog = (lane_blocks_data(script), table, start)
+        script._batched_program = prog
+    return prog
+
+
 class BatchedRun:
     """Lanes of one script in one persistent launch.
 
@@ -124,11 +163,11 @@ class BatchedRun:
         rio = s.reg_io_state_size
         st = np.zeros((self.G, 2, rio * 32), np.uint32)
         st[:, :, self.flag_bits.astype(np.int64)] = 0xFFFFFFFF
-        table, start = gather_table(s)
+        lane_bd, table, start = batched_program(s)
         nz = lambda a: a if a.size else np.zeros(1, a.dtype)
         self.d = dict(
             bs=to_device(np.ascontiguousarray(s.blocks_start, np.uint64)),
-            bd=to_device(np.ascontiguousarray(s.blocks_data, np.uint32)),
+            bd=to_device(lane_bd),
             gtab=to_device(table),
             gstart=to_device(start),
             states=to_device(st.reshape(-1)),

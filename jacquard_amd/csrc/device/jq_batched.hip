@@ -44,7 +44,7 @@ namespace {
 
 constexpr int kLayerWords = 8192;
 constexpr int kSecWords = 5120;  // 4096 permutation words + 256 x 4 flags
-constexpr int kLdsWords = 3 * kLayerWords + 2 * kSecWords;
+constexpr int kLdsWords = 2 * kLayerWords + 2 * kSecWords;
 constexpr u32 kNoSource = 0xFFFFFFFFu;
 
 struct BArgs {
@@ -98,6 +98,9 @@ __device__ __forceinline__ u32 perm_at(const u32 *sec, u32 p) {
   return (b & 1) ? (w >> 16) : (w & 0xffffu);
 }
 
+// bit b of a flag word as an all-lanes mask
+__device__ __forceinline__ u32 bmask(u32 word, int b) { return 0u - ((word >> b) & 1u); }
+
 // flag bit of position p as an all-lanes mask
 __device__ __forceinline__ u32 fbit(u32 word, u32 p) {
   return 0u - ((word >> (p & 31)) & 1u);
@@ -139,9 +142,9 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
                                 u32 *ram_group, u32 *lds, u32 *prof) {
   const u32 t = threadIdx.x;
   const u32 lane = t & 63;
-  u32 *bufA = lds, *bufB = lds + kLayerWords, *WO = lds + 2 * kLayerWords;
+  u32 *layer = lds, *WO = lds + kLayerWords;
   // section buffer k & 1 (pointer arithmetic, not an array: no scratch)
-  auto secb = [lds](u32 k) { return lds + 3 * kLayerWords + (k & 1u) * kSecWords; };
+  auto secb = [lds](u32 k) { return lds + 2 * kLayerWords + (k & 1u) * kSecWords; };
   size_t pi = 0;
   for (;; gsrc += kLayerWords) {
     const u32 S = script[pi];
@@ -170,88 +173,96 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
       for (int j = 0; j < 32; ++j) {
         u32 e = src[j], v = 0;
         if (e != kNoSource) v = ((e >> 31) ? out : in)[e & 0x7fffffffu];
-        bufA[sw(t + 256 * j)] = v;
+        layer[sw(t + 256 * j)] = v;
       }
     }
     sec_wait();
     __syncthreads();
 
-    u32 *prev = bufA, *cur = bufB;
+    // ---- boomerang stages, in place on the one layer: every thread first
+    // gathers its 32 level-1 operands into registers, then (after a
+    // barrier) the layer is overwritten with this stage's after-layout
     for (u32 s = 0; s < S; ++s) {
-      const u32 *sec = secb(s);
-      const u32 *fl = sec + 4096;
+      const uint4 *S4 = reinterpret_cast<const uint4 *>(secb(s));
+      const uint4 P0 = S4[t], P1 = S4[256 + t], P2 = S4[512 + t], P3 = S4[768 + t];
+      const uint4 F = S4[1024 + t];
       // the next section goes into the other buffer, last read by the
       // previous stage, which finished before the last barrier
       sec_fetch(secg + (size_t)kSecWords * (s + 1), secb(s + 1));
-      // levels 1..5 of the subtree rooted at position 256 + t
+      const u32 pw[16] = {P0.x, P0.y, P0.z, P0.w, P1.x, P1.y, P1.z, P1.w,
+                          P2.x, P2.y, P2.z, P2.w, P3.x, P3.y, P3.z, P3.w};
+      u32 g[32];
+#pragma unroll
+      for (int m = 0; m < 32; ++m) g[m] = layer[(pw[m >> 1] >> (16 * (m & 1))) & 0xffffu];
+      __syncthreads();  // all operands read before the layer is rewritten
+      // levels 1..5 of the subtree rooted at position 256 + t; gate g's
+      // flags are bit g of F.x / F.y / F.z
       u32 L1[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
-        u32 j = t + 256u * k;
-        u32 a = prev[sw(perm_at(sec, j))];
-        u32 b = prev[sw(perm_at(sec, j + 4096))];
-        L1[k] = lgate(a, b, flags_at(fl, 4096 + j));
-        cur[sw(4096 + j)] = L1[k];
+        L1[k] = (g[k] ^ bmask(F.x, k)) & ((g[k + 16] ^ bmask(F.y, k)) | bmask(F.z, k));
+        layer[sw(4096 + t + 256u * k)] = L1[k];
       }
       u32 L2[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        u32 p = 2048 + t + 256u * k;
-        L2[k] = lgate(L1[k], L1[k + 8], flags_at(fl, p));
-        cur[sw(p)] = L2[k];
+        L2[k] = (L1[k] ^ bmask(F.x, 16 + k)) & ((L1[k + 8] ^ bmask(F.y, 16 + k)) |
+                                                 bmask(F.z, 16 + k));
+        layer[sw(2048 + t + 256u * k)] = L2[k];
       }
       u32 L3[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        u32 p = 1024 + t + 256u * k;
-        L3[k] = lgate(L2[k], L2[k + 4], flags_at(fl, p));
-        cur[sw(p)] = L3[k];
+        L3[k] = (L2[k] ^ bmask(F.x, 24 + k)) & ((L2[k + 4] ^ bmask(F.y, 24 + k)) |
+                                                 bmask(F.z, 24 + k));
+        layer[sw(1024 + t + 256u * k)] = L3[k];
       }
       u32 L4[2];
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        u32 p = 512 + t + 256u * k;
-        L4[k] = lgate(L3[k], L3[k + 2], flags_at(fl, p));
-        cur[sw(p)] = L4[k];
+        L4[k] = (L3[k] ^ bmask(F.x, 28 + k)) & ((L3[k + 2] ^ bmask(F.y, 28 + k)) |
+                                                 bmask(F.z, 28 + k));
+        layer[sw(512 + t + 256u * k)] = L4[k];
       }
-      cur[sw(256 + t)] = lgate(L4[0], L4[1], flags_at(fl, 256 + t));
+      layer[sw(256 + t)] = (L4[0] ^ bmask(F.x, 30)) & ((L4[1] ^ bmask(F.y, 30)) | bmask(F.z, 30));
       __syncthreads();
       if (t < 64) {
+        // wave 0 lane L: flags of positions 128+L, 192+L, 64+L, L in F.w
+        const u32 fw = F.w;
+        auto wg = [fw](u32 a, u32 b, int i) {
+          return (a ^ bmask(fw, 3 * i)) & ((b ^ bmask(fw, 3 * i + 1)) | bmask(fw, 3 * i + 2));
+        };
         // level 6: positions 128+L and 192+L; level 7: 64+L
-        u32 o6a = lgate(cur[sw(256 + lane)], cur[sw(384 + lane)], flags_at(fl, 128 + lane));
-        u32 o6b = lgate(cur[sw(320 + lane)], cur[sw(448 + lane)], flags_at(fl, 192 + lane));
-        u32 o7 = lgate(o6a, o6b, flags_at(fl, 64 + lane));
+        u32 o6a = wg(layer[sw(256 + lane)], layer[sw(384 + lane)], 0);
+        u32 o6b = wg(layer[sw(320 + lane)], layer[sw(448 + lane)], 1);
+        u32 o7 = wg(o6a, o6b, 2);
         // level 8: lanes 32..63 -> position L from L+32 (lane L-32) and
         // L+64 (own level-7 word)
         u32 c = 0;
-        const Flags f0 = flags_at(fl, lane);
         u32 a8 = __shfl(o7, (lane + 32) & 63);
-        if (lane >= 32) c = lgate(a8, o7, f0);
+        if (lane >= 32) c = wg(a8, o7, 3);
 #pragma unroll
         for (int len = 16; len >= 1; len >>= 1) {
           u32 a = __shfl(c, (lane + len) & 63);
           u32 b = __shfl(c, (lane + 2 * len) & 63);
-          if (lane >= (u32)len && lane < 2u * len) c = lgate(a, b, f0);
+          if (lane >= (u32)len && lane < 2u * len) c = wg(a, b, 3);
         }
-        cur[sw(128 + lane)] = o6a;
-        cur[sw(192 + lane)] = o6b;
-        cur[sw(64 + lane)] = o7;
-        cur[sw(lane)] = c;  // lane 0 keeps position 0 at zero
+        layer[sw(128 + lane)] = o6a;
+        layer[sw(192 + lane)] = o6b;
+        layer[sw(64 + lane)] = o7;
+        layer[sw(lane)] = c;  // lane 0 keeps position 0 at zero
       }
       sec_wait();  // section s+1 is read right after this barrier
       __syncthreads();
       if ((hook >> 8) == s) {
         const u32 src = 32u * (hook & 255u);
 #pragma unroll
-        for (u32 q = 0; q < 32; ++q) WO[sw(32 * t + q)] = cur[sw(src + q)];
+        for (u32 q = 0; q < 32; ++q) WO[sw(32 * t + q)] = layer[sw(src + q)];
       }
-      u32 *tmp = prev;
-      prev = cur;
-      cur = tmp;
     }
     __syncthreads();
 
-    // ---- SRAM + duplicate permutation into bufA (free now)
+    // ---- SRAM + duplicate permutation into the layer (free now)
     const u32 nperm = ns * 4 + nd;
     {
       const u32 *sec = secb(S);
@@ -261,7 +272,7 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
       for (u32 q = t; q < nperm * 32; q += 256) {
         uint4 f = *reinterpret_cast<const uint4 *>(fl + 4 * (q >> 5));
         u32 v = WO[sw(perm_at(sec, q))];
-        bufA[sw(q)] = (v & ~fbit(f.y, q)) ^ fbit(f.x, q);
+        layer[sw(q)] = (v & ~fbit(f.y, q)) ^ fbit(f.x, q);
       }
     }
     __syncthreads();
@@ -275,12 +286,12 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
         const u32 P = 128 * s;
         u32 ra = 0, wa = 0, en = 0, data = 0;
         for (int k = 0; k < 13; ++k) {
-          ra |= ((bufA[sw(P + k)] >> l) & 1u) << k;
-          wa |= ((bufA[sw(P + 16 + k)] >> l) & 1u) << k;
+          ra |= ((layer[sw(P + k)] >> l) & 1u) << k;
+          wa |= ((layer[sw(P + 16 + k)] >> l) & 1u) << k;
         }
         for (int k = 0; k < 32; ++k) {
-          en |= ((bufA[sw(P + 32 + k)] >> l) & 1u) << k;
-          data |= ((bufA[sw(P + 64 + k)] >> l) & 1u) << k;
+          en |= ((layer[sw(P + 32 + k)] >> l) & 1u) << k;
+          data |= ((layer[sw(P + 64 + k)] >> l) & 1u) << k;
         }
         u32 *ram = ram_group + (size_t)l * A.sram_size + so + (size_t)s * kSramWords;
         r = ram[ra];
@@ -297,7 +308,7 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
       if (act) WO[sw(32 * (num_ios - ns + s) + l)] = mine;
     }
     for (u32 q = t; q < nd * 32; q += 256)
-      WO[sw(32 * (num_ios - ns - nd) + q)] = bufA[sw(ns * 128 + q)];
+      WO[sw(32 * (num_ios - ns - nd) + q)] = layer[sw(ns * 128 + q)];
     sec_wait();  // the clock-enable section is read after this barrier
     __syncthreads();
 

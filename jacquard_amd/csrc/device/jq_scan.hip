@@ -150,6 +150,15 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
     const size_t p_ck = p_sd + 5120;
     uint4 q0 = ld4(script + p_st + 4 * t), q1 = ld4(script + p_st + 1024 + 4 * t);
     uint4 q2 = ld4(script + p_st + 2048 + 4 * t), q3 = ld4(script + p_st + 3072 + 4 * t);
+    // wave 0's tree flags of stage 0 (words 0..255 of the flag block)
+    uint4 fA = make_uint4(0, 0, 0, 0), fB = fA, fC = fA, fD = fA;
+    if (w0) {
+      const u32 *fl0 = script + p_st + 4096;
+      fA = ld4(fl0 + 4 * lane);
+      fB = ld4(fl0 + 4 * (64 + lane));
+      fC = ld4(fl0 + 4 * (128 + lane));
+      fD = ld4(fl0 + 4 * (192 + lane));
+    }
     const uint4 sd0 = ld4(script + p_sd + 4 * t), sd1 = ld4(script + p_sd + 1024 + 4 * t);
     const uint4 sd2 = ld4(script + p_sd + 2048 + 4 * t), sd3 = ld4(script + p_sd + 3072 + 4 * t);
     const uint4 sdf = ld4(script + p_sd + 4096 + 4 * t);
@@ -210,20 +219,22 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
     for (u32 bs = 0; bs < num_stages; ++bs) {
       u32 hv = gather_bits(st_v, q0, q1, q2, q3);
       u32 hx = is_x ? gather_bits(st_x, q0, q1, q2, q3) : 0u;
-      const u32 *fl = script + pi + 4096;
-      uint4 fA, fB, fC, fD;
-      if (w0) {
-        fA = ld4(fl + 4 * lane);
-        fB = ld4(fl + 4 * (64 + lane));
-        fC = ld4(fl + 4 * (128 + lane));
-        fD = ld4(fl + 4 * (192 + lane));
-      }
       pi += 5120;
+      // next stage's permutation and tree flags are loaded now, a stage
+      // ahead of their use; this stage's came with the previous one
+      uint4 nA = fA, nB = fB, nC = fC, nD = fD;
       if (bs + 1 < num_stages) {
         q0 = ld4(script + pi + 4 * t);
         q1 = ld4(script + pi + 1024 + 4 * t);
         q2 = ld4(script + pi + 2048 + 4 * t);
         q3 = ld4(script + pi + 3072 + 4 * t);
+        if (w0) {
+          const u32 *fn = script + pi + 4096;
+          nA = ld4(fn + 4 * lane);
+          nB = ld4(fn + 4 * (64 + lane));
+          nC = ld4(fn + 4 * (128 + lane));
+          nD = ld4(fn + 4 * (192 + lane));
+        }
       }
       hin_v[t] = hv;
       hin_x[t] = hx;
@@ -326,6 +337,10 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
         if (is_x) wo_x[t] = st_x[hook & 255];
         if (TIMING) wo_arr[t] = st_arr[hook & 255];
       }
+      fA = nA;
+      fB = nB;
+      fC = nC;
+      fD = nD;
     }
     __syncthreads();
 

@@ -6,6 +6,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "aig.hpp"
 #include "compile.hpp"
@@ -473,6 +474,73 @@ int jqh_part_features(const uint32_t *bd, uint64_t n_bd, const uint64_t *parts,
       f[22] = (float)bits / 256.f;
       f[23] = num_ios ? (float)hooks / num_ios : 0.f;
       std::memcpy(out + 24 * p, f, sizeof(f));
+    }
+  });
+}
+
+
+// Batched-form layout of every boomerang-stage section of a script: out is
+// a copy of blocks_data in which each stage section (4096 permutation words
+// + 256 x {xora, xorb, orb, delay}, src/flatten.rs:1027-1063) is replaced
+// by the lane layout read by the batched kernel:
+//   words 1024*(w>>2) + 4t + (w&3), w < 16: thread t's operand indices
+//     m = 2w (low u16) and 2w+1 (high u16), the row-swizzled LDS index
+//     q ^ ((q>>5)&31) of previous-layer position q = perm[t + 256m];
+//   words 4096 + 4t + c, c < 3: xora/xorb/orb of thread t's 31 subtree
+//     gates at bit g: level-1 position 4096+t+256g (g < 16), level 2
+//     2048+t+256(g-16) (g < 24), level 3 1024+... (g < 28), level 4
+//     512+... (g < 30), level 5 256+t (g = 30);
+//   word 4096 + 4t + 3, t < 64: flags of positions 128+t, 192+t, 64+t, t
+//     at bits 3i..3i+2.
+int jqh_lane_blocks_data(const uint32_t *bd, uint64_t n_bd, const uint64_t *parts,
+                         uint64_t n, uint32_t *out) {
+  return guard([&] {
+    std::memcpy(out, bd, n_bd * sizeof(uint32_t));
+    std::vector<uint64_t> secs;
+    for (uint64_t p = 0; p < n; ++p) {
+      uint64_t off = parts[5 * p + 3];
+      if (off + 256 > n_bd) throw std::runtime_error("lane layout: bad offset");
+      uint32_t S = bd[off], R = bd[off + 6];
+      uint64_t base = off + 256 + 512ull * R;
+      if (base + 5120ull * (S + 2) > n_bd) throw std::runtime_error("lane layout: truncated");
+      for (uint32_t k = 0; k < S; ++k) secs.push_back(base + 5120ull * k);
+    }
+    auto perm = [](const uint32_t *sec, uint32_t q) -> uint32_t {
+      uint32_t w = sec[1024 * ((q & 31) >> 3) + 4 * (q >> 5) + ((q >> 1) & 3)];
+      return (q & 1) ? (w >> 16) : (w & 0xffffu);
+    };
+    auto flag = [](const uint32_t *sec, uint32_t q, int c) -> uint32_t {
+      return (sec[4096 + 4 * (q >> 5) + c] >> (q & 31)) & 1u;
+    };
+    auto sw = [](uint32_t q) { return q ^ ((q >> 5) & 31u); };
+    static const uint32_t kBase[5] = {4096, 2048, 1024, 512, 256};
+    static const uint32_t kCnt[5] = {16, 8, 4, 2, 1};
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)secs.size(); ++i) {
+      const uint32_t *sec = bd + secs[i];
+      uint32_t *o = out + secs[i];
+      for (uint32_t t = 0; t < 256; ++t) {
+        for (uint32_t w = 0; w < 16; ++w) {
+          uint32_t lo = sw(perm(sec, t + 256 * (2 * w)));
+          uint32_t hi = sw(perm(sec, t + 256 * (2 * w + 1)));
+          o[1024 * (w >> 2) + 4 * t + (w & 3)] = lo | (hi << 16);
+        }
+        uint32_t f[3] = {0, 0, 0};
+        uint32_t g = 0;
+        for (int lv = 0; lv < 5; ++lv)
+          for (uint32_t k = 0; k < kCnt[lv]; ++k, ++g)
+            for (int c = 0; c < 3; ++c) f[c] |= flag(sec, kBase[lv] + t + 256 * k, c) << g;
+        uint32_t w3 = 0;
+        if (t < 64) {
+          const uint32_t pos[4] = {128 + t, 192 + t, 64 + t, t};
+          for (int j = 0; j < 4; ++j)
+            for (int c = 0; c < 3; ++c) w3 |= flag(sec, pos[j], c) << (3 * j + c);
+        }
+        o[4096 + 4 * t + 0] = f[0];
+        o[4096 + 4 * t + 1] = f[1];
+        o[4096 + 4 * t + 2] = f[2];
+        o[4096 + 4 * t + 3] = w3;
+      }
     }
   });
 }
