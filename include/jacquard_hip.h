@@ -166,6 +166,23 @@ int jq_simulate_batched_profiled(
  * scan kernel (kind 0) or the batched kernel (kind 1) on the current device. */
 int jq_max_resident_blocks(int kind, int *out_blocks);
 
+/* ---- Output change extraction (no counterpart in the reference) ---------
+ * Replaces the host-side scan of every snapshot in write_output_vcd_impl
+ * (src/sim/vcd_io.rs:758-820): for each of num_snapshots emitted snapshots
+ * (word offsets snapshot_offsets[k] into states, in emission order), the
+ * outputs whose value differs from the previous emitted snapshot's (every
+ * output at k = 0).  output_bits[i] is output i's state bit, 0xFFFFFFFF for
+ * a constant 0 or 0xFFFFFFFE for a constant 1; a value is 3 (X) when bit
+ * xmask_offset*32 + state bit of the snapshot is set (xmask_offset = 0: no X
+ * lane).  With records == NULL, writes counts[k]; with records, writes the
+ * changes of snapshot k in output order at records[record_offsets[k] ...]
+ * as (i << 2 | value).  Records are compacted on the device, so only they
+ * need to cross PCIe. */
+int jq_output_changes(const u32 *states, const usize *snapshot_offsets,
+                      u32 num_snapshots, const u32 *output_bits, u32 num_outputs,
+                      u32 xmask_offset, u32 *counts, u32 *records,
+                      const usize *record_offsets, void *stream);
+
 /* ---- Learned cost model MLP (no counterpart in the reference) -----------
  * Fully connected FP32 network in -> h1 -> h2 -> 1 with ReLU (in <= 32,
  * h1, h2 <= 64), row-major weights W1[h1][in], b1[h1], W2[h2][h1], b2[h2],

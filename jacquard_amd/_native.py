@@ -102,6 +102,7 @@ HIP_SYMBOLS = [
     "jq_simulate_batched",
     "jq_simulate_batched_profiled",
     "jq_max_resident_blocks",
+    "jq_output_changes",
     "jq_mlp_forward",
     "jq_mlp_train_step",
     "jq_mlp_num_params",
@@ -139,6 +140,8 @@ def hip():
             c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp, c_vp, c_vp, c_vp,
             c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp, c_u32, c_vp, c_u32,
             c_vp]),
+        "jq_output_changes": (ctypes.c_int, [c_vp, c_vp, c_u32, c_vp, c_u32,
+                                             c_u32, c_vp, c_vp, c_vp, c_vp]),
         "jq_max_resident_blocks": (ctypes.c_int, [ctypes.c_int,
                                                   P(ctypes.c_int)]),
         "jq_mlp_forward": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int,

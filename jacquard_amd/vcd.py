@@ -344,6 +344,73 @@ def write_output_vcd(script, offsets_timestamps, states, timescale=(1, "ps"),
     return "\n".join(lines) + "\n"
 
 
+CONST0_BIT = 0xFFFFFFFF
+CONST1_BIT = 0xFFFFFFFE
+
+
+def output_changes_device(script, d_states, offsets_timestamps, stream=None):
+    """Per emitted snapshot, the outputs whose value changed, computed on the
+    device by jq_output_changes over the device snapshot buffer (laid out
+    [values | X | arrivals] per snapshot, effective_state_size words each).
+    offsets_timestamps are parse_input_vcd's (value-only offsets); skipped
+    timestamps are left out, as the writer skips them.  Returns (counts
+    uint32[n], records uint32[sum]): record = output << 2 | value (3 = X)."""
+    import ctypes
+    import torch
+    from . import _native as N
+    from .sim import to_device
+    mapping = output_mapping(script)
+    rio = script.reg_io_state_size
+    ess = script.effective_state_size()
+    snaps = [off // rio for off, ts in offsets_timestamps if ts != U64_MAX]
+    bits = np.asarray([CONST1_BIT if (pos is None and lit == 1) else
+                       CONST0_BIT if pos is None else pos
+                       for lit, pos, _ in mapping], np.uint32)
+    n = len(snaps)
+    if n == 0:
+        return np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+    d_off = to_device(np.asarray(snaps, np.uint64) * np.uint64(ess))
+    d_bits = to_device(bits if bits.size else np.zeros(1, np.uint32))
+    d_cnt = torch.zeros(n, dtype=torch.int32, device="cuda")
+    xoff = rio if script.xprop_enabled else 0
+    st = stream if stream is not None else torch.cuda.current_stream()
+    L = N.hip()
+    N.check(L.jq_output_changes(d_states.data_ptr(), d_off.data_ptr(), n, d_bits.data_ptr(),
+                                len(bits), xoff, d_cnt.data_ptr(), None, None,
+                                ctypes.c_void_p(st.cuda_stream)))
+    cnt64 = d_cnt.to(torch.int64)
+    rec_off = torch.cumsum(cnt64, 0) - cnt64
+    total = int(cnt64.sum().item())
+    d_rec = torch.zeros(max(total, 1), dtype=torch.int32, device="cuda")
+    N.check(L.jq_output_changes(d_states.data_ptr(), d_off.data_ptr(), n, d_bits.data_ptr(),
+                                len(bits), xoff, None, d_rec.data_ptr(),
+                                rec_off.data_ptr(), ctypes.c_void_p(st.cuda_stream)))
+    torch.cuda.synchronize()
+    return (d_cnt.cpu().numpy().view(np.uint32),
+            d_rec[:total].cpu().numpy().view(np.uint32))
+
+
+def write_output_vcd_changes(script, offsets_timestamps, counts, records,
+                             timescale=(1, "ps"), scope="gem_top_module"):
+    """Output VCD text from per-snapshot change records
+    (output_changes_device); the same text write_output_vcd produces from
+    the full snapshots."""
+    mapping = output_mapping(script)
+    lines = _header_text(timescale, scope, mapping)
+    codes = [id_code(i) for i in range(len(mapping))]
+    k = 0
+    r = 0
+    for _, ts in offsets_timestamps:
+        if ts == U64_MAX:
+            continue
+        lines.append(f"#{ts}")
+        for rec in records[r:r + int(counts[k])].tolist():
+            lines.append(f"{_SYM[rec & 3]}{codes[rec >> 2]}")
+        r += int(counts[k])
+        k += 1
+    return "\n".join(lines) + "\n"
+
+
 def write_output_vcd_timed(script, offsets_timestamps, states, arrival_states,
                            timescale=(1, "ps"), scope="gem_top_module",
                            xmask_states=None):

@@ -57,6 +57,24 @@ def test_golden_vcd_gpu(case):
     assert out == gold
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_golden_vcd_gpu_change_records(case):
+    """The same flow with the output changes extracted on the device
+    (jq_output_changes) instead of scanning host snapshots."""
+    d, gv, iv, gold = case
+    base = os.path.join(HERE, d)
+    nl = Netlist.from_verilog(os.path.join(base, gv))
+    s = Compiled(nl).flatten(2)
+    with open(os.path.join(base, iv)) as f:
+        states, offs, hdr = vcd.parse_input_vcd(f.read(), s)
+    out = sim.run_scan(s, states, len(offs), return_device=True)
+    cnt, rec = vcd.output_changes_device(s, out["device"]["states"], offs)
+    text = vcd.write_output_vcd_changes(s, offs, cnt, rec, hdr.timescale)
+    with open(os.path.join(base, gold)) as f:
+        assert text == f.read()
+
+
 def test_input_vcd_quirks():
     """Snapshot count and timestamps of the dff_test stimulus
     (src/sim/vcd_io.rs:347-455)."""

@@ -129,3 +129,28 @@ def test_xprop_timing_srams_together():
     assert np.array_equal(out["sram_x"][:s.sram_storage_size], sram_x)
     assert out["event_count"] == n_ref and n_ref > 0
     assert np.array_equal(sim.canonical_events(out["events"]), sim.canonical_events(ev_ref))
+
+
+@pytest.mark.parametrize("xprop", [False, True])
+def test_device_output_changes_match_host_writer(xprop):
+    """VCD text from device change records equals the host writer's text
+    from the full snapshots (values and X), on a random design with many
+    outputs and skipped timestamps."""
+    from jacquard_amd import vcd
+    nl, comp = design("xp", level_split=8, seed=5, gates=20000, depth=40,
+                      dffs=2000, inputs=64, outputs=32, geometric_p=0.3, srams=2)
+    s = comp.flatten(16, xprop=xprop)
+    cycles = 70
+    m = stimulus.dff_initial_xmask(s, 0.2, seed=3)[0] if xprop else None
+    st = stimulus.build_states(s, cycles, seed=17, p=0.4, initial_xmask=m)
+    out = sim.run_scan(s, st.copy(), cycles, return_device=True)
+    rio, ess = s.reg_io_state_size, s.effective_state_size()
+    offs = [((k + 1) * rio, vcd.U64_MAX if k % 7 == 3 else 10 * k) for k in range(cycles)]
+    S = out["states"].reshape(-1, ess)
+    values = np.ascontiguousarray(S[:, :rio]).reshape(-1)
+    xm = np.ascontiguousarray(S[:, rio:2 * rio]).reshape(-1) if xprop else None
+    host = vcd.write_output_vcd(s, offs, values, xmask_states=xm)
+    cnt, rec = vcd.output_changes_device(s, out["device"]["states"], offs)
+    dev = vcd.write_output_vcd_changes(s, offs, cnt, rec)
+    assert dev == host
+    assert len(rec) < S[1:].size  # compacted: far fewer words than snapshots
