@@ -43,12 +43,15 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cycles-per-step", type=int, default=10_000)
-    ap.add_argument("--workload", choices=["plain", "xtiming", "lanes4m"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "xtiming", "lanes4m", "mult16m"],
+                    default="plain",
                     help="plain: the 1M-gate 2-state drop-in (the headline line); "
                          "xtiming: the 4M-gate netlist with 8 SRAMs, 10%% initial-X "
                          "DFFs, 20-200 ps gate delays and 50/20 ps setup/hold at 5 ns; "
                          "lanes4m: batched+learned lanes vs the plain drop-in on that "
-                         "4M-gate netlist (value lanes)")
+                         "4M-gate netlist (value lanes); mult16m: the 16M-gate structured "
+                         "netlist (64 16x16 multiplier tiles + 2M-gate control AIG, 1M "
+                         "DFF), --lanes lanes of --cycles cycles, learned vs heuristic")
     ap.add_argument("--gates", type=int, default=None)
     ap.add_argument("--depth", type=int, default=None)
     ap.add_argument("--dffs", type=int, default=None)
@@ -181,7 +184,7 @@ def bench_plain(a, script, torch):
     return out
 
 
-def bench_batched(a, comp, torch):
+def bench_batched(a, comp, torch, cycles=None):
     """Batched form on the same netlist: --lanes lanes with per-lane seeds,
     cycle counts U[10K, 200K] x --batched-scale and toggle rates
     U[0.02, 0.5], in one persistent launch with the device lane queue,
@@ -195,7 +198,8 @@ def bench_batched(a, comp, torch):
     lo, hi = int(10_000 * a.batched_scale), int(200_000 * a.batched_scale)
     seeds = rng.integers(1, 2**31, a.lanes).astype(np.uint32)
     probs = rng.uniform(0.02, 0.5, a.lanes)
-    cycles = rng.integers(max(lo, 1), max(hi, 1) + 1, a.lanes)
+    if cycles is None:
+        cycles = rng.integers(max(lo, 1), max(hi, 1) + 1, a.lanes)
     tune = None
     if a.batched_blocks:
         W = a.batched_blocks
@@ -452,6 +456,75 @@ def bench_lanes4m(a, torch):
     }), flush=True)
 
 
+def bench_mult16m(a, torch):
+    """The structured 16M-gate netlist: 64 replicated 16x16 AND-array
+    multiplier tiles with pipeline DFFs plus a 2M-gate random control AIG,
+    depth 128, 1M DFFs, level split 8.  --lanes lanes (64) of --cycles cycles
+    (20K) run batched with the block assignment from the learned cost model
+    (trained on >=50K samples of the kernel's own partition timings) and,
+    for the makespan comparison, with the heuristic one; blocks per group
+    default to the co-resident maximum for the lane groups (no width sweep:
+    each flattening of this script takes tens of seconds).  The plain
+    drop-in runs --steps x --cycles-per-step cycles beside it."""
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    from jacquard_amd import batched as B
+    import ctypes
+    from jacquard_amd import _native as N
+    t = time.time()
+    nl = Netlist.multiplier(seed=a.seed)
+    t_gen = time.time() - t
+    t = time.time()
+    comp = Compiled(nl, level_split=a.split)
+    t_comp = time.time() - t
+    log(f"mult16m: netlist {nl.counts()} generated in {t_gen:.1f}s, compiled in "
+        f"{t_comp:.1f}s; " + comp.log.strip().splitlines()[-1])
+    groups = a.batched_groups or (a.lanes + 31) // 32
+    if not a.batched_blocks:
+        a.batched_blocks = max(8, min(max(comp.parts_per_stage()),
+                                      B.max_resident_batched() // groups))
+    a.batched_groups = groups
+    cycles = np.full(a.lanes, a.cycles)
+    b = bench_batched(a, comp, torch, cycles=cycles)
+    value = b["lane_cycles"] / b["wall"]
+    res = ctypes.c_int(0)
+    N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
+    script = comp.flatten(min(max(comp.parts_per_stage()), res.value))
+    p = bench_plain(a, script, torch)
+    plain = p["cycles"] / p["wall"]
+    bs = b["script"]
+    W_bytes = 4 * int(bs.blocks_start[-1])
+    per_lane = W_bytes / 32 + (bs.algorithmic_bytes_per_cycle() - W_bytes)
+    achieved = per_lane * b["lane_cycles"] / (b["kernel_ms"] / 1e3) / 1e9
+    print(json.dumps({
+        "metric": METRIC, "value": value, "unit": "lane-cycles/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": b["wall"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic",
+        "config": {
+            "workload": f"batched+learned on the structured netlist: "
+                        f"{nl.counts()['num_and']} AND (64 16x16 multiplier tiles + "
+                        f"2M-gate control AIG), {nl.counts()['num_dffs']} DFF; "
+                        f"{a.lanes} lanes x {a.cycles} cycles",
+            "host_s": {"generate": t_gen, "compile": t_comp},
+            "batched": {"lanes": a.lanes, "groups": b["groups"],
+                        "blocks_per_group": b["blocks"], "lane_cycles": b["lane_cycles"],
+                        "learned_cost_model": b["learned"],
+                        "launch": "one hipGraph replay (reset + persistent kernel)",
+                        "verified_cycles": b["verified_cycles"]},
+            "plain": {"lane_cycles_per_s": plain, "blocks": script.num_blocks,
+                      "verified_cycles": p.get("verified_cycles")},
+            "vs_plain": value / plain,
+            "major_stages": bs.num_major_stages, "partitions": bs.num_parts,
+            "script_bytes": int(bs.blocks_data.nbytes)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "batched_kernel",
+                     "algorithmic_bytes_per_lane_cycle": per_lane},
+        "cpu_baseline": None,
+    }), flush=True)
+
+
 def cpu_baseline(a, script):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -502,6 +575,8 @@ def main():
         copy_calibration(a.copy_calibration)
     if a.workload == "lanes4m":
         return bench_lanes4m(a, torch)
+    if a.workload == "mult16m":
+        return bench_mult16m(a, torch)
     if a.workload == "xtiming":
         x = bench_xtiming(a, torch)
         s = x["script"]

@@ -85,6 +85,8 @@ def host():
                                              c_vp]),
         "jqh_lane_blocks_data": (ctypes.c_int, [c_vp, c_u64, c_vp, c_u64,
                                                 c_vp]),
+        "jqh_gather_table": (ctypes.c_int, [c_vp, c_u64, c_vp, c_u64, c_u32,
+                                            c_u32, c_vp, c_vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -21,7 +21,8 @@ NO_SOURCE = 0xFFFFFFFF
 
 
 def gather_table(script):
-    """Per-partition gather table of the batched form.
+    """Per-partition gather table of the batched form (jqh_gather_table in
+    libjacquard_host.so).
 
     For partition p (blocks_data order, the rows of ``script.parts``) the
     table holds 8192 u32: entry q is the state bit that lands at position q
@@ -36,37 +37,18 @@ def gather_table(script):
     Returns (table uint32[num_parts * 8192], start uint32[major * blocks]):
     start[m*num_blocks + b] = index of the first partition of block b in
     major stage m."""
-    bd = script.blocks_data
-    parts = np.asarray(script.parts)
+    bd = np.ascontiguousarray(script.blocks_data, np.uint32)
+    parts = np.ascontiguousarray(script.parts, np.uint64)
     n = len(parts)
-    table = np.full((max(n, 1), 8192), NO_SOURCE, np.uint32)
-    k32 = np.arange(32, dtype=np.uint32)
-    tt = np.arange(256, dtype=np.int64)
-    for p in range(n):
-        off = int(parts[p, 3])
-        R = int(bd[off + 6])
-        if R == 0:
-            continue
-        gr = bd[off + 256:off + 256 + 512 * R].reshape(R, 256, 2)
-        idx = gr[:, :, 0].T                     # [thread, round]
-        mask = gr[:, :, 1].T
-        bits = ((mask[:, :, None] >> k32) & 1).astype(bool)   # [t, r, k]
-        src = ((idx[:, :, None] & np.uint32(0x7FFFFFFF)) * np.uint32(32) + k32) \
-            | (idx[:, :, None] & np.uint32(0x80000000))
-        bits = bits.reshape(256, -1)
-        src = src.reshape(256, -1)
-        rank = np.cumsum(bits, axis=1) - 1
-        cnt = bits.sum(axis=1)
-        pos = 32 * tt[:, None] + cnt[:, None] - 1 - rank
-        table[p, pos[bits]] = src[bits]
+    table = np.empty(max(n, 1) * 8192, np.uint32)
+    if n == 0:
+        table[:] = NO_SOURCE
     start = np.zeros(script.num_major_stages * script.num_blocks, np.uint32)
-    seen = set()
-    for k in range(n):
-        key = int(parts[k, 0]) * script.num_blocks + int(parts[k, 1])
-        if key not in seen:
-            seen.add(key)
-            start[key] = k
-    return table.reshape(-1), start
+    if N.host().jqh_gather_table(bd.ctypes.data, bd.size, parts.ctypes.data, n,
+                                 script.num_blocks, script.num_major_stages,
+                                 table.ctypes.data, start.ctypes.data) != 0:
+        raise RuntimeError(N.host_error())
+    return table, start
 
 
 def _lane_gate_positions():

@@ -545,4 +545,49 @@ int jqh_lane_blocks_data(const uint32_t *bd, uint64_t n_bd, const uint64_t *part
   });
 }
 
+
+// Gather table of the batched form: for partition p (rows of `parts`, in
+// blocks_data order) 8192 u32, entry q = the state bit that lands at
+// first-layer position q after the global read (bit 31: from the current
+// snapshot), 0xFFFFFFFF where the position stays zero.  Thread t's
+// (idx, mask) entries of rounds 0..R-1 are taken in order, each mask's set
+// bits ascending; the n-th of its B bits lands at position 32t + B-1-n
+// (csrc/kernel_v1_impl.cuh:87-151).  start[m*num_blocks + b] = index of the
+// first partition of block b in major stage m (0 for an empty block).
+int jqh_gather_table(const uint32_t *bd, uint64_t n_bd, const uint64_t *parts, uint64_t n,
+                     uint32_t num_blocks, uint32_t num_major, uint32_t *table,
+                     uint32_t *start) {
+  return guard([&] {
+    for (uint64_t k = 0; k < (uint64_t)num_blocks * num_major; ++k) start[k] = 0;
+    for (uint64_t p = n; p-- > 0;) {
+      uint64_t key = parts[5 * p + 0] * num_blocks + parts[5 * p + 1];
+      if (key >= (uint64_t)num_blocks * num_major)
+        throw std::runtime_error("gather table: bad partition row");
+      start[key] = (uint32_t)p;  // the lowest index wins
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < (int64_t)n; ++p) {
+      uint32_t *tab = table + (size_t)p * 8192;
+      for (int q = 0; q < 8192; ++q) tab[q] = 0xFFFFFFFFu;
+      uint64_t off = parts[5 * p + 3];
+      uint32_t R = bd[off + 6];
+      const uint32_t *g = bd + off + 256;
+      for (uint32_t t = 0; t < 256; ++t) {
+        uint32_t B = 0;
+        for (uint32_t r = 0; r < R; ++r) B += (uint32_t)__builtin_popcount(g[512 * r + 2 * t + 1]);
+        uint32_t nth = 0;
+        for (uint32_t r = 0; r < R; ++r) {
+          uint32_t idx = g[512 * r + 2 * t], m = g[512 * r + 2 * t + 1];
+          while (m) {
+            uint32_t k = (uint32_t)__builtin_ctz(m);
+            m &= m - 1;
+            tab[32 * t + (B - 1 - nth)] = (((idx & 0x7FFFFFFFu) * 32u + k) | (idx & 0x80000000u));
+            ++nth;
+          }
+        }
+      }
+    }
+  });
+}
+
 }  // extern "C"

@@ -5,12 +5,17 @@
 #include "compile.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace jq {
 
@@ -260,6 +265,11 @@ void endpoint_outputs(const Aig &g, const MajorStage &ms, size_t e,
 void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
                         std::string *log) {
   std::ostringstream lg;
+  using clk = std::chrono::steady_clock;
+  double t_group = 0, t_map = 0;
+  auto secs = [](clk::time_point a) {
+    return std::chrono::duration<double>(clk::now() - a).count();
+  };
   Levels L = levelize(g);
   uint32_t split = opt.level_split ? opt.level_split : std::max<uint32_t>(1, L.max_lvl);
   uint32_t nmajor = std::max<uint32_t>(1, (L.max_lvl + split - 1) / split);
@@ -294,14 +304,23 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
   }
   for (size_t e = 0; e < ne; ++e) out.stages[ep_stage[e]].orig_endpoints.push_back((uint32_t)e);
 
-  Scratch sc(g.num_pins + 1);
   std::vector<uint32_t> mark_all(g.num_pins + 1, 0);
   std::vector<uint32_t> seen(g.num_pins + 1, 0);
   uint32_t gstamp = 1, sstamp = 1;
   std::vector<uint32_t> lvl_m(g.num_pins + 1, 0);
   size_t total_parts = 0, total_bstages = 0, total_nodes = 0;
 
+  // per-thread mapping contexts (stamp arrays over all pins), made on first use
+  struct MapCtx {
+    Scratch sc;
+    std::vector<uint32_t> mark, seen, stack;
+    uint32_t gstamp = 1, sstamp = 1;
+    explicit MapCtx(size_t n) : sc(n), mark(n, 0), seen(n, 0) {}
+  };
+  std::vector<std::unique_ptr<MapCtx>> ctxs;
+
   for (uint32_t m = 0; m < nmajor; ++m) {
+    auto t0 = clk::now();
     MajorStage &ms = out.stages[m];
     auto avail_pin = [&](uint32_t p) { return stage_of(p) < (int)m; };
     // stage-relative levels (cone depth above this stage's inputs)
@@ -392,9 +411,17 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
       cur_srams += infos[e].nsram;
     }
     flush();
+    t_group += secs(t0);
+    t0 = clk::now();
 
-    // map every group; split in halves on failure
-    std::function<void(std::vector<uint32_t>)> build = [&](std::vector<uint32_t> eps) {
+    // map every group (in parallel, each thread with its own stamp arrays);
+    // split a group in halves when it does not fit
+    std::vector<std::vector<Partition>> built(group_eps.size());
+    std::function<void(MapCtx &, std::vector<uint32_t>, std::vector<Partition> &)> build =
+        [&](MapCtx &cx, std::vector<uint32_t> eps, std::vector<Partition> &res) {
+      Scratch &sc = cx.sc;
+      std::vector<uint32_t> &mark = cx.mark, &seen = cx.seen, &stack = cx.stack;
+      uint32_t &gstamp = cx.gstamp, &sstamp = cx.sstamp;
       MapInput mi;
       ++sstamp;
       uint32_t nsram = 0;
@@ -466,18 +493,53 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
           throw std::runtime_error("compile: a single endpoint does not fit a partition");
         std::vector<uint32_t> a(eps.begin(), eps.begin() + eps.size() / 2);
         std::vector<uint32_t> b(eps.begin() + eps.size() / 2, eps.end());
-        build(a);
-        build(b);
+        build(cx, a, res);
+        build(cx, b, res);
         return;
       }
       Partition part;
       part.endpoints = eps;
       part.stages = std::move(stages);
-      total_bstages += part.stages.size();
-      total_nodes += mi.nodes.size();
-      ms.parts.push_back(std::move(part));
+      part.num_nodes = mi.nodes.size();
+      res.push_back(std::move(part));
     };
-    for (auto &eps : group_eps) build(eps);
+    std::string par_err;
+#pragma omp parallel
+    {
+      int tid = 0;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+#pragma omp single
+      {
+        int nt = 1;
+#ifdef _OPENMP
+        nt = omp_get_num_threads();
+#endif
+        if ((int)ctxs.size() < nt) ctxs.resize(nt);
+      }
+#pragma omp for schedule(dynamic, 1)
+      for (int64_t gi = 0; gi < (int64_t)group_eps.size(); ++gi) {
+        try {
+          if (!ctxs[tid]) {
+            auto c = std::make_unique<MapCtx>(g.num_pins + 1);
+            ctxs[tid] = std::move(c);
+          }
+          build(*ctxs[tid], group_eps[gi], built[gi]);
+        } catch (const std::exception &e) {
+#pragma omp critical(jq_compile_err)
+          if (par_err.empty()) par_err = e.what();
+        }
+      }
+    }
+    if (!par_err.empty()) throw std::runtime_error(par_err);
+    for (auto &v : built)
+      for (auto &part : v) {
+        total_bstages += part.stages.size();
+        total_nodes += part.num_nodes;
+        ms.parts.push_back(std::move(part));
+      }
+    t_map += secs(t0);
     // sort by decreasing number of boomerang stages (src/pe.rs:1068)
     std::stable_sort(ms.parts.begin(), ms.parts.end(),
                      [](const Partition &x, const Partition &y) {
@@ -488,6 +550,7 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
        << ms.orig_endpoints.size() << " endpoints, " << ms.parts.size()
        << " partitions\n";
   }
+  lg << "compile time: grouping " << t_group << " s, mapping " << t_map << " s\n";
   lg << "total: " << total_parts << " partitions, " << total_bstages
      << " boomerang stages, " << total_nodes << " mapped AND evaluations ("
      << g.num_and() << " unique)\n";

@@ -35,6 +35,7 @@ struct BoomStage {
 struct Partition {
   std::vector<uint32_t> endpoints;  // indices into MajorStage endpoint space
   std::vector<BoomStage> stages;
+  size_t num_nodes = 0;             // AND gates mapped into it
 };
 
 // One level-split slice of the AIG (mirrors StagedAIG, src/staging.rs:12-21).
