@@ -236,6 +236,291 @@ bool map_partition(const Aig &g, const MapInput &in, Scratch &sc,
 }
 
 // ---------------------------------------------------------------------------
+// Multi-level boomerang mapper.  A boomerang stage is a full binary tree:
+// node (k, q) at level k >= 1 (q < 8192 >> k) reads its children (k-1, q)
+// and (k-1, q + (8192 >> k)); level 0 is the shuffled layer, whose slots can
+// hold any value present in the previous layer.  A node either computes the
+// AND of its children (with inversions) or passes child (k-1, q) through.
+// A gate whose cone, down to values already present, is d levels deep can
+// therefore be computed at level d in one stage, with its operand gates
+// recomputed at its children (every root owns its subtree; a value with
+// fanout is simply computed again in each subtree that needs it) and
+// present values routed up by pass-through chains.  Per stage:
+//   1. present values that are outputs not yet written: level-1
+//      pass-throughs at the front (dense write-out words);
+//   2. roots in critical-path order at their cone depth (outputs at the
+//      front of their level, others from the back);
+//   3. present values still needed by gates not computed yet: level-1
+//      pass-throughs; if they do not fit, roots are taken back, last first;
+//   4. the 32-bit after-layout words holding written outputs are hooked.
+// Every node of levels 1..13 is in the after-layout and readable by the
+// next stage.
+bool map_partition_ml(const Aig &g, const MapInput &in, Scratch &sc,
+                      std::vector<BoomStage> &out, uint32_t max_write_words,
+                      uint32_t max_level) {
+  out.clear();
+  std::vector<uint32_t> pins;
+  pins.reserve(in.inputs.size() + in.nodes.size());
+  for (auto p : in.inputs) pins.push_back(p);
+  for (auto p : in.nodes) pins.push_back(p);
+  const uint32_t ni = (uint32_t)in.inputs.size();
+  const uint32_t n = (uint32_t)pins.size();
+  for (uint32_t i = 0; i < n; ++i) sc.loc[pins[i]] = (int32_t)i;
+  struct Guard {
+    Scratch &s;
+    const std::vector<uint32_t> &p;
+    ~Guard() {
+      for (auto x : p) s.loc[x] = -1;
+    }
+  } guard{sc, pins};
+  std::vector<uint32_t> opa(n, 0), opb(n, 0);
+  std::vector<std::vector<uint32_t>> cons(n);
+  for (uint32_t i = ni; i < n; ++i) {
+    uint32_t p = pins[i];
+    int32_t la = sc.loc[g.a[p] >> 1], lb = sc.loc[g.b[p] >> 1];
+    if (la < 0 || lb < 0) return false;
+    opa[i] = (uint32_t)la;
+    opb[i] = (uint32_t)lb;
+    cons[la].push_back(i);
+    if (lb != la) cons[lb].push_back(i);
+  }
+  std::vector<uint8_t> is_out(n, 0), written(n, 0), done(n, 0), present(n, 0);
+  for (auto p : in.outputs) {
+    int32_t l = sc.loc[p];
+    if (l < 0) return false;
+    is_out[l] = 1;
+  }
+  std::vector<uint32_t> height(n, 0);
+  for (uint32_t i = n; i-- > ni;)
+    for (auto c : cons[i]) height[i] = std::max(height[i], height[c] + 1);
+  for (uint32_t i = 0; i < ni; ++i) done[i] = present[i] = 1;
+  constexpr uint32_t kInf = 0xFFFFFFFFu;
+  auto capl = [](int k) -> uint32_t { return k == 0 ? (uint32_t)kWidth : (uint32_t)kWidth >> k; };
+  std::vector<std::vector<uint32_t>> occ(kBoomStages + 1);
+  std::vector<uint32_t> depth(n, kInf), now(n, 0);
+  std::vector<std::pair<uint8_t, uint16_t>> log;
+  // place value x at node (k, q) with its whole subtree
+  std::function<bool(uint32_t, int, uint32_t)> put = [&](uint32_t x, int k, uint32_t q) -> bool {
+    if (occ[k][q] != kNone) return false;
+    if (k == 0) {
+      if (!present[x]) return false;
+      occ[0][q] = x;
+      log.push_back({0, (uint16_t)q});
+      return true;
+    }
+    if (!present[x] && (depth[x] == kInf || depth[x] > (uint32_t)k)) return false;
+    occ[k][q] = x;
+    log.push_back({(uint8_t)k, (uint16_t)q});
+    if (present[x]) return put(x, k - 1, q);
+    return put(opa[x], k - 1, q) && put(opb[x], k - 1, q + capl(k));
+  };
+  // undo a placement attempt that failed (its nodes were never counted)
+  auto rollback = [&](size_t mark) {
+    while (log.size() > mark) {
+      auto e = log.back();
+      log.pop_back();
+      occ[e.first][e.second] = kNone;
+    }
+  };
+  auto commit_now = [&](size_t mark) {
+    for (size_t j = mark; j < log.size(); ++j) {
+      uint32_t x = occ[log[j].first][log[j].second];
+      if (log[j].first > 0 && !present[x]) ++now[x];
+    }
+  };
+  uint32_t total_words = 0;
+  while (true) {
+    bool todo = false;
+    for (uint32_t i = ni; i < n && !todo; ++i) todo = !done[i];
+    for (uint32_t i = 0; i < n && !todo; ++i) todo = is_out[i] && !written[i];
+    if (!todo) break;
+    if (out.size() >= 4096) return false;
+    for (int k = 0; k <= kBoomStages; ++k) occ[k].assign(capl(k), kNone);
+    std::fill(now.begin(), now.end(), 0u);
+    log.clear();
+    // cone depth of every gate not computed yet, over present values
+    for (uint32_t i = 0; i < ni; ++i) depth[i] = present[i] ? 0 : kInf;
+    for (uint32_t i = ni; i < n; ++i) {
+      if (done[i]) {
+        depth[i] = present[i] ? 0 : kInf;
+        continue;
+      }
+      uint32_t da = present[opa[i]] ? 0 : depth[opa[i]];
+      uint32_t db = present[opb[i]] ? 0 : depth[opb[i]];
+      depth[i] = (da == kInf || db == kInf) ? kInf : 1 + std::max(da, db);
+    }
+    std::vector<std::pair<uint8_t, uint16_t>> wpos;  // written outputs
+    struct Root {
+      uint32_t x;
+      size_t mark, end;
+      bool out;
+      uint8_t k;
+      uint16_t q;
+    };
+    std::vector<Root> roots;
+    std::vector<uint8_t> wrote_now(n, 0);
+    auto reset_stage = [&]() {
+      for (int k = 0; k <= kBoomStages; ++k) std::fill(occ[k].begin(), occ[k].end(), kNone);
+      std::fill(now.begin(), now.end(), 0u);
+      log.clear();
+      wpos.clear();
+      roots.clear();
+      std::fill(wrote_now.begin(), wrote_now.end(), 0);
+    };
+    // 1. unwritten present outputs: level-1 pass-throughs at the front
+    auto place_present_outputs = [&]() -> bool {
+      uint32_t front1 = 0;
+      for (uint32_t i = 0; i < n; ++i) {
+        if (!(is_out[i] && !written[i] && present[i])) continue;
+        while (front1 < capl(1) && (occ[1][front1] != kNone || occ[0][front1] != kNone)) ++front1;
+        if (front1 >= capl(1) || !put(i, 1, front1)) return false;
+        wpos.push_back({1, (uint16_t)front1});
+        wrote_now[i] = 1;
+      }
+      return true;
+    };
+    // first-fit root placement: outputs from the front of their level,
+    // other roots from the back
+    auto place_root = [&](uint32_t x, uint32_t max_tries) -> bool {
+      int k = (int)depth[x];
+      uint32_t cap = capl(k);
+      size_t mark = log.size();
+      uint32_t tries = 0;
+      for (uint32_t j = 0; j < cap && tries < max_tries; ++j) {
+        uint32_t q = is_out[x] ? j : cap - 1 - j;
+        if (occ[k][q] != kNone) continue;
+        ++tries;
+        if (put(x, k, q)) {
+          roots.push_back({x, mark, log.size(), (bool)is_out[x], (uint8_t)k, (uint16_t)q});
+          commit_now(mark);
+          return true;
+        }
+        rollback(mark);
+      }
+      return false;
+    };
+    // 3. carries: present values still needed by gates not computed yet
+    auto needed_later = [&](uint32_t v) {
+      for (auto c : cons[v])
+        if (!done[c] && !now[c]) return true;
+      return false;
+    };
+    std::vector<uint8_t> at_l1(n, 0);
+    auto carries_fit = [&](bool place) -> bool {
+      std::fill(at_l1.begin(), at_l1.end(), 0);
+      for (int k = 1; k <= kBoomStages; ++k)
+        for (auto x : occ[k])
+          if (x != kNone) at_l1[x] = 1;
+      uint32_t need = 0;
+      for (uint32_t v = 0; v < n; ++v) need += present[v] && !at_l1[v] && needed_later(v);
+      uint32_t free_slots = 0;
+      for (uint32_t q = 0; q < capl(1); ++q)
+        free_slots += occ[1][q] == kNone && occ[0][q] == kNone;
+      if (need > free_slots) return false;
+      if (place) {
+        uint32_t q = capl(1);
+        for (uint32_t v = 0; v < n; ++v) {
+          if (!(present[v] && !at_l1[v] && needed_later(v))) continue;
+          while (q-- > 0)
+            if (occ[1][q] == kNone && occ[0][q] == kNone) break;
+          put(v, 1, q);
+        }
+      }
+      return true;
+    };
+    // 2. the deepest D whose whole frontier (gates within depth D that are
+    // outputs or feed deeper gates) fits, with its carries
+    uint32_t dmax = 0;
+    for (uint32_t i = ni; i < n; ++i)
+      if (!done[i] && depth[i] != kInf) dmax = std::max(dmax, depth[i]);
+    bool accepted = false;
+    for (uint32_t D = std::min(dmax, max_level); D >= 2 && !accepted; --D) {
+      std::vector<uint32_t> front;
+      for (uint32_t i = ni; i < n; ++i) {
+        if (done[i] || depth[i] == kInf || depth[i] > D) continue;
+        bool f = is_out[i];
+        for (auto c : cons[i])
+          f |= !done[c] && (depth[c] == kInf || depth[c] > D);
+        if (f) front.push_back(i);
+      }
+      std::stable_sort(front.begin(), front.end(), [&](uint32_t x, uint32_t y) {
+        if (depth[x] != depth[y]) return depth[x] > depth[y];
+        return height[x] > height[y];
+      });
+      reset_stage();
+      if (!place_present_outputs()) return false;
+      bool ok = true;
+      for (uint32_t x : front)
+        if (!place_root(x, 0xFFFFFFFFu)) {
+          ok = false;
+          break;
+        }
+      if (ok && carries_fit(true)) accepted = true;
+    }
+    if (!accepted) {
+      // depth-1 roots by priority, as many as fit with their carries
+      reset_stage();
+      if (!place_present_outputs()) return false;
+      std::vector<uint32_t> cand;
+      for (uint32_t i = ni; i < n; ++i)
+        if (!done[i] && depth[i] == 1) cand.push_back(i);
+      std::stable_sort(cand.begin(), cand.end(),
+                       [&](uint32_t x, uint32_t y) { return height[x] > height[y]; });
+      for (uint32_t x : cand) place_root(x, 64);
+      while (!carries_fit(false)) {
+        if (roots.empty()) return false;
+        Root r = roots.back();
+        roots.pop_back();
+        for (size_t j = r.end; j-- > r.mark;) {
+          auto e = log[j];
+          uint32_t x = occ[e.first][e.second];
+          if (x == kNone) continue;
+          if (e.first > 0 && !present[x]) --now[x];
+          occ[e.first][e.second] = kNone;
+        }
+      }
+      carries_fit(true);
+    }
+    // progress: something computed or written
+    if (wpos.empty() && roots.empty()) return false;
+    for (auto &r : roots)
+      if (r.out) wpos.push_back({r.k, r.q});
+    // 4. hooks
+    BoomStage bs;
+    for (int k = 0; k <= kBoomStages; ++k) {
+      bool any = false;
+      for (auto x : occ[k]) any |= x != kNone;
+      if (!any) continue;
+      bs.hier[k].assign(capl(k), kNone);
+      for (uint32_t q = 0; q < capl(k); ++q)
+        if (occ[k][q] != kNone) bs.hier[k][q] = pins[occ[k][q]];
+    }
+    std::vector<uint16_t> words;
+    for (auto &w : wpos) words.push_back((uint16_t)((capl(w.first) + w.second) >> 5));
+    std::sort(words.begin(), words.end());
+    words.erase(std::unique(words.begin(), words.end()), words.end());
+    bs.write_outs = words;
+    total_words += (uint32_t)words.size();
+    if (total_words > max_write_words) return false;
+    // advance: computed gates, written outputs, present set of the next stage
+    for (auto &r : roots)
+      if (r.out) written[r.x] = 1;
+    for (uint32_t i = 0; i < n; ++i)
+      if (wrote_now[i]) written[i] = 1;
+    for (uint32_t i = ni; i < n; ++i)
+      if (now[i]) done[i] = 1;
+    std::fill(present.begin(), present.end(), 0);
+    for (int k = 1; k <= kBoomStages; ++k)
+      for (auto x : occ[k])
+        if (x != kNone) present[x] = 1;
+    // an output computed inside a subtree but not as a root is written by a
+    // later stage (step 1), so it must stay present: it is, at level >= 1
+    out.push_back(std::move(bs));
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // Endpoint bookkeeping for one major stage.
 struct EndpointInfo {
   std::vector<uint32_t> outs;  // pins that must be written out
@@ -486,8 +771,13 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
       bool ok = mi.inputs.size() <= kWidth && nsram * 4 + dup_words <= 256 &&
                 nsram + dup_words < 256;
       std::vector<BoomStage> stages;
-      if (ok)
-        ok = map_partition(g, mi, sc, stages, 256 - nsram - dup_words);
+      if (ok) {
+        // multi-level mapping first, the level-1 mapper as the fallback
+        bool m = opt.multilevel &&
+                 map_partition_ml(g, mi, sc, stages, 256 - nsram - dup_words, opt.multilevel);
+        if (!m) m = map_partition(g, mi, sc, stages, 256 - nsram - dup_words);
+        ok = m;
+      }
       if (!ok) {
         if (eps.size() <= 1)
           throw std::runtime_error("compile: a single endpoint does not fit a partition");

@@ -58,6 +58,9 @@ struct CompileOptions {
   uint32_t max_outputs = 2048;
   uint32_t max_level_nodes = 2600;
   uint32_t max_part_nodes = 24000;
+  // highest tree level at which the mapper places AND gates (0: level-1
+  // mapper only, one logic level per boomerang stage)
+  uint32_t multilevel = 8;
 };
 
 struct Script {

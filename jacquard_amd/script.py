@@ -24,10 +24,13 @@ class Compiled:
     per-partition cost vector."""
 
     def __init__(self, netlist: Netlist, level_split=0, max_inputs=0,
-                 max_outputs=0, max_level_nodes=0, max_part_nodes=0):
+                 max_outputs=0, max_level_nodes=0, max_part_nodes=0, multilevel=8):
+        """multilevel: highest boomerang-tree level at which AND gates are
+        placed (0 = one logic level per boomerang stage)."""
         self.netlist = netlist
         h = N.host().jqh_compile(netlist.handle, level_split, max_inputs,
-                                 max_outputs, max_level_nodes, max_part_nodes)
+                                 max_outputs, max_level_nodes, max_part_nodes,
+                                 multilevel)
         if not h:
             raise RuntimeError("compile failed: " + N.host_error())
         self._h = h
