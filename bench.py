@@ -64,7 +64,11 @@ def parse():
     ap.add_argument("--outputs", type=int, default=256)
     ap.add_argument("--toggle", type=float, default=0.1)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--split", type=int, default=4, help="level-split period")
+    ap.add_argument("--split", type=int, default=6,
+                    help="level-split period of the plain drop-in's script")
+    ap.add_argument("--batched-split", type=int, default=4,
+                    help="level-split period of the batched form's script (it runs "
+                         "more partitions per block, so it prefers shorter major stages)")
     ap.add_argument("--blocks", type=int, default=0, help="plain blocks (0 = auto)")
     ap.add_argument("--lanes", type=int, default=256)
     ap.add_argument("--batched-scale", type=float, default=0.02,
@@ -419,6 +423,8 @@ def bench_lanes4m(a, torch):
     p = bench_plain(a, script, torch)
     plain = p["cycles"] / p["wall"]
     log(f"lanes4m: plain {plain:,.0f} cycles/s")
+    if a.batched_split != a.split:
+        comp = Compiled(nl, level_split=a.batched_split)
     b = bench_batched(a, comp, torch)
     value = b["lane_cycles"] / b["wall"]
     bs = b["script"]
@@ -628,7 +634,9 @@ def main():
     log(f"plain: {value:,.0f} lane-cycles/s, launch {avg_launch_s * 1e3:.1f} ms")
     forms = {"plain": {"lane_cycles_per_s": value, "verified_cycles": p.get("verified_cycles")}}
     if not a.no_batched:
-        b = bench_batched(a, comp, torch)
+        from jacquard_amd.script import Compiled
+        comp_b = comp if a.batched_split == a.split else Compiled(nl, level_split=a.batched_split)
+        b = bench_batched(a, comp_b, torch)
         forms["batched"] = {"lane_cycles_per_s": b["lane_cycles"] / b["wall"],
                             "lanes": a.lanes, "groups": b["groups"],
                             "blocks_per_group": b["blocks"],

@@ -466,18 +466,40 @@ bool map_partition_ml(const Aig &g, const MapInput &in, Scratch &sc,
         if (!done[i] && depth[i] == 1) cand.push_back(i);
       std::stable_sort(cand.begin(), cand.end(),
                        [&](uint32_t x, uint32_t y) { return height[x] > height[y]; });
-      for (uint32_t x : cand) place_root(x, 64);
-      while (!carries_fit(false)) {
-        if (roots.empty()) return false;
-        Root r = roots.back();
-        roots.pop_back();
-        for (size_t j = r.end; j-- > r.mark;) {
-          auto e = log[j];
-          uint32_t x = occ[e.first][e.second];
-          if (x == kNone) continue;
-          if (e.first > 0 && !present[x]) --now[x];
-          occ[e.first][e.second] = kNone;
+      for (uint32_t x : cand) place_root(x, 0xFFFFFFFFu);
+      if (!carries_fit(false)) {
+        // too many carries (wide inputs): defer roots, lowest priority
+        // first, whose operands stay carried anyway; each frees one level-1
+        // slot without adding a carry
+        std::fill(at_l1.begin(), at_l1.end(), 0);
+        for (int k = 1; k <= kBoomStages; ++k)
+          for (auto x : occ[k])
+            if (x != kNone) at_l1[x] = 1;
+        uint32_t need = 0, free_slots = 0;
+        for (uint32_t v = 0; v < n; ++v) need += present[v] && !at_l1[v] && needed_later(v);
+        for (uint32_t q = 0; q < capl(1); ++q)
+          free_slots += occ[1][q] == kNone && occ[0][q] == kNone;
+        for (size_t r = roots.size(); r-- > 0 && need > free_slots;) {
+          const uint32_t x = roots[r].x;
+          bool free_defer = true;
+          for (uint32_t v : {opa[x], opb[x]}) {
+            if (!present[v] || at_l1[v]) continue;
+            bool other = false;
+            for (auto c : cons[v]) other |= c != x && !done[c] && !now[c];
+            if (!other) free_defer = false;
+          }
+          if (!free_defer) continue;
+          for (size_t j = roots[r].end; j-- > roots[r].mark;) {
+            auto e = log[j];
+            uint32_t y = occ[e.first][e.second];
+            if (y == kNone) continue;
+            if (e.first > 0 && !present[y]) --now[y];
+            occ[e.first][e.second] = kNone;
+          }
+          roots.erase(roots.begin() + (long)r);
+          ++free_slots;
         }
+        if (need > free_slots) return false;
       }
       carries_fit(true);
     }
