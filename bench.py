@@ -13,7 +13,11 @@ The same run also measures the batched lane-sliced form (256 lanes of the
 same netlist, per-lane seeds, cycle counts U[10K, 200K] scaled by
 --batched-scale, toggle rates U[0.02, 0.5]) and times the reference CPU
 path (the oracle's literal restatement of sanity_check_cpu) on a bounded
-cycle sample on one host core.
+cycle sample on one host core, with, labelled separately, the same
+restatement spread over the host's cores (OpenMP) and the direct AIG
+simulator.  Beside the HBM roofline it reports the LDS roofline (the
+interpreter's LDS bytes against the peak measured by an LDS probe kernel)
+and the grid-barrier time per major stage.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -31,6 +35,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "simulated lane-cycles/sec (batched+learned vs plain GPU vs CPU), bit-exact"
 HBM_PEAK_GBPS = 8000.0
+LDS_SPEC_PEAK_GBPS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
 
 
 def log(*a):
@@ -549,6 +554,114 @@ def cpu_baseline(a, script):
     return dict(value=n / spent, cycles=n, seconds=spent, state_words=ss)
 
 
+def cpu_variants(a, script):
+    """The two CPU figures reported beside the one-thread port: the same
+    restatement with the blocks of each major stage spread over the host's
+    cores (OpenMP), and the direct AIG cycle simulator (no script) on the
+    same netlist.  Each on a bounded cycle sample of about a third of
+    --cpu-seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from jacquard_amd import stimulus
+    threads = max(1, min(16, os.cpu_count() or 1))
+    budget = a.cpu_seconds / 3
+    out = {}
+
+    def bounded(run):
+        n, spent = 8, 0.0
+        while True:
+            spent = run(n)
+            if spent >= budget / 4 or n >= 4096:
+                return n, spent
+            n = min(4096, max(n * 2, int(n * (budget / max(spent, 1e-3)))))
+
+    def omp(n):
+        st = stimulus.build_states(script, n, seed=a.seed, p=a.toggle)
+        t = time.perf_counter()
+        oracle.run_v1(script, st, n, threads=threads)
+        return time.perf_counter() - t
+
+    n, s = bounded(omp)
+    out["port_openmp"] = {"lane_cycles_per_s": n / s, "threads": threads, "cycles": n,
+                          "seconds": s}
+    A = script.netlist.arrays()
+    data, fl = stimulus.input_roles(script)
+    pos2pin = {v: k for k, v in script.input_map.items() if A["drv"][k] in (2, 3)}
+    pins = [pos2pin[p] for p in data.tolist()] + [pos2pin[p] for p in fl.tolist()]
+
+    def direct(n):
+        vals = stimulus.lane_inputs(len(data), n, a.seed, a.toggle)
+        bits = np.concatenate([vals[:n].astype(np.uint8),
+                               np.ones((n, len(fl)), np.uint8)], axis=1)
+        t = time.perf_counter()
+        oracle.aig_sim(A, n, pins, bits, A["pos"])
+        return time.perf_counter() - t
+
+    n, s = bounded(direct)
+    out["direct_aig"] = {"lane_cycles_per_s": n / s, "threads": 1, "cycles": n,
+                         "seconds": s}
+    return out
+
+
+def lds_peak_gbps(torch):
+    """Measured LDS read peak: the probe kernel (8 workgroups per CU, each
+    streaming 64 KB of LDS with conflict-free 16-byte reads), best of 5
+    launches timed with HIP events on the launch stream."""
+    import ctypes
+    from jacquard_amd import _native as N
+    L = N.hip()
+    blocks = torch.cuda.get_device_properties(0).multi_processor_count * 8
+    iters = 2048
+    sink = torch.empty(blocks * 256, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    N.check(L.jq_lds_read_probe(blocks, 64, sink.data_ptr(), sp))
+    best = None
+    for _ in range(5):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        N.check(L.jq_lds_read_probe(blocks, iters, sink.data_ptr(), sp))
+        e.record(stream)
+        e.synchronize()
+        ms = s.elapsed_time(e)
+        best = ms if best is None else min(best, ms)
+    return blocks * 256 * 128 * iters / (best / 1e3) / 1e9
+
+
+def barrier_us_per_stage(script, torch, cycles=2000):
+    """Time of one major stage of the scan kernel when every block of the
+    script is empty (a zero header): the grid barrier's latency at this
+    block count, the limiter that is neither HBM nor LDS."""
+    import ctypes
+    from jacquard_amd import _native as N
+    from jacquard_amd import sim
+    L = N.hip()
+    B, M = script.num_blocks, script.num_major_stages
+    bs = np.arange(B * M + 1, dtype=np.uint64) * 256
+    d_bs = sim.to_device(bs)
+    d_bd = torch.zeros(B * M * 256 + 2048, dtype=torch.int32, device="cuda")
+    st = torch.zeros(cycles + 1, dtype=torch.int32, device="cuda")
+    one = torch.zeros(1, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def launch():
+        N.check(L.jq_simulate_scan(B, M, d_bs.data_ptr(), d_bd.data_ptr(),
+                                   one.data_ptr(), one.data_ptr(), cycles, 1,
+                                   st.data_ptr(), None, None, 0, 0,
+                                   ctypes.c_void_p(stream.cuda_stream)))
+
+    launch()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record(stream)
+    for _ in range(3):
+        launch()
+    e.record(stream)
+    e.synchronize()
+    if sim.device_error():
+        raise RuntimeError("device barrier timeout during the barrier measurement")
+    return s.elapsed_time(e) / 3 / (cycles * M) * 1e3
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "scan_kernel_1M_gates_hbm_traffic.json")
 
 
@@ -633,6 +746,14 @@ def main():
     traffic = traffic_from_profile(a.cycles_per_step, int(script.blocks_data.nbytes))
     log(f"plain: {value:,.0f} lane-cycles/s, launch {avg_launch_s * 1e3:.1f} ms")
     forms = {"plain": {"lane_cycles_per_s": value, "verified_cycles": p.get("verified_cycles")}}
+    lds_bpc = script.algorithmic_lds_bytes_per_cycle()
+    lds_peak = lds_peak_gbps(torch)
+    lds_achieved = lds_bpc * a.cycles_per_step / avg_launch_s / 1e9
+    log(f"LDS: {lds_achieved:,.0f} GB/s of a measured {lds_peak:,.0f} GB/s peak")
+    bar_us = barrier_us_per_stage(script, torch)
+    stage_us = avg_launch_s * 1e6 / (a.cycles_per_step * script.num_major_stages)
+    log(f"barrier: {bar_us:.2f} us per major stage with empty blocks, "
+        f"{stage_us:.2f} us per major stage in the run")
     if not a.no_batched:
         from jacquard_amd.script import Compiled
         comp_b = comp if a.batched_split == a.split else Compiled(nl, level_split=a.batched_split)
@@ -663,6 +784,10 @@ def main():
                          f"{c['seconds']:.1f} s"}
         forms["cpu"] = {"lane_cycles_per_s": c["value"]}
         log(f"cpu: {c['value']:.2f} lane-cycles/s over {c['cycles']} cycles")
+        forms["cpu"].update(cpu_variants(a, script))
+        log(f"cpu: OpenMP {forms['cpu']['port_openmp']['lane_cycles_per_s']:.2f} "
+            f"({forms['cpu']['port_openmp']['threads']} threads), direct AIG "
+            f"{forms['cpu']['direct_aig']['lane_cycles_per_s']:.2f} lane-cycles/s")
     line = {
         "metric": METRIC,
         "value": value,
@@ -698,6 +823,20 @@ def main():
             "traffic": traffic,
             "kernel": "scan_kernel",
             "algorithmic_bytes_per_cycle": bpc,
+        },
+        "lds_roofline": {
+            "achieved": lds_achieved,
+            "peak": lds_peak,
+            "unit": "GB/s",
+            "frac": lds_achieved / lds_peak,
+            "peak_source": "measured: jq_lds_read_probe, 8 workgroups per CU",
+            "spec_peak": LDS_SPEC_PEAK_GBPS,
+            "algorithmic_bytes_per_cycle": lds_bpc,
+        },
+        "barrier": {
+            "us_per_major_stage_empty": bar_us,
+            "us_per_major_stage_run": stage_us,
+            "blocks": script.num_blocks,
         },
         "cpu_baseline": cpu,
     }

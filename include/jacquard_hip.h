@@ -183,6 +183,15 @@ int jq_output_changes(const u32 *states, const usize *snapshot_offsets,
                       u32 xmask_offset, u32 *counts, u32 *records,
                       const usize *record_offsets, void *stream);
 
+/* ---- LDS read-bandwidth probe (no counterpart in the reference) ---------
+ * Measures the device's LDS read peak, the figure the interpreter kernels'
+ * LDS roofline is quoted against: num_blocks workgroups of 256 threads each
+ * read 64 KB of LDS iters times with conflict-free 16-byte reads
+ * (num_blocks * 256 * 128 * iters bytes in all) and write one checksum word
+ * per thread to sink (u32[num_blocks * 256]).  Asynchronous on `stream`;
+ * the caller times it. */
+int jq_lds_read_probe(u32 num_blocks, u32 iters, u32 *sink, void *stream);
+
 /* ---- Learned cost model MLP (no counterpart in the reference) -----------
  * Fully connected FP32 network in -> h1 -> h2 -> 1 with ReLU (in <= 32,
  * h1, h2 <= 64), row-major weights W1[h1][in], b1[h1], W2[h2][h1], b2[h2],

@@ -105,6 +105,7 @@ HIP_SYMBOLS = [
     "jq_simulate_batched_profiled",
     "jq_max_resident_blocks",
     "jq_output_changes",
+    "jq_lds_read_probe",
     "jq_mlp_forward",
     "jq_mlp_train_step",
     "jq_mlp_num_params",
@@ -144,6 +145,7 @@ def hip():
             c_vp]),
         "jq_output_changes": (ctypes.c_int, [c_vp, c_vp, c_u32, c_vp, c_u32,
                                              c_u32, c_vp, c_vp, c_vp, c_vp]),
+        "jq_lds_read_probe": (ctypes.c_int, [c_u32, c_u32, c_vp, c_vp]),
         "jq_max_resident_blocks": (ctypes.c_int, [ctypes.c_int,
                                                   P(ctypes.c_int)]),
         "jq_mlp_forward": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int,

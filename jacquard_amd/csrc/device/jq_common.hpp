@@ -67,6 +67,83 @@ __device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen, u32 *err) {
   __syncthreads();
 }
 
+// Agent-coherent access to memory that other workgroups of the launch read
+// or write (relaxed agent-scope atomics: on gfx950 loads and stores with the
+// device-scope cache policy, which bypass the non-coherent cached copies).
+__device__ __forceinline__ u32 ld_coherent(const u32 *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coherent(u32 *p, u32 v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Wait until this wave's memory operations have completed.
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Grid barrier for kernels whose inter-workgroup data moves only through
+// ld_coherent / st_coherent.  Those accesses are coherent at agent scope by
+// themselves, so the barrier needs ordering, not cache maintenance: every
+// wave waits for its stores to complete, then the workgroup arrives; after
+// the generation changes, the workgroup's coherent loads are issued and see
+// every store made before any arrival.  This drops the agent-scope release
+// fence (an L2 writeback) and acquire fence (an L2 invalidate) that
+// grid_sync issues per workgroup, which serialise in each XCD's L2.
+//
+// A slot is kCohSlotWords u32: the generation at [32] and two counter sets
+// at 256 and 2304, used by alternate barriers, so the last arrivers' resets
+// need no wait (a set is reused two barriers later, after the resetting
+// workgroup's next arrival has waited for its stores).  A set holds a top
+// counter at [0] and, for n >= 128, eight per-XCD sub-counters at [128 *
+// (1 + x)] (workgroups are dispatched round-robin over the 8 XCDs, x =
+// blockIdx % 8): one contended atomic per arrival costs ~10 ns at the
+// coherence point, so 156 flat arrivals take ~1.5 us, two levels ~0.3 us.
+// Measured empty major stage at 156 workgroups: 5.2 us with grid_sync, 2.1 us
+// with this barrier.  Bounded wait as in grid_sync.
+constexpr int kCohSlots = 64;
+constexpr int kCohSlotWords = 4096;
+
+__device__ __forceinline__ void grid_sync_coherent(u32 *bar, u32 n, u32 &gen, u32 *err) {
+  wait_vmem();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 *set = bar + 256 + ((gen + 1u) & 1u) * 2048;
+    bool last = false;
+    if (n >= 128) {
+      const u32 x = blockIdx.x & 7u;
+      const u32 nx = (n - x + 7u) >> 3;  // workgroups with blockIdx % 8 == x
+      u32 a = __hip_atomic_fetch_add(&set[128 * (1 + x)], 1u, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+      if (a == nx - 1u) {
+        __hip_atomic_store(&set[128 * (1 + x)], 0u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        u32 top = __hip_atomic_fetch_add(&set[0], 1u, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        last = top == 7u;
+      }
+    } else {
+      u32 a = __hip_atomic_fetch_add(&set[0], 1u, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+      last = a == n - 1u;
+    }
+    if (last) {
+      __hip_atomic_store(&set[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bar[32], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      u32 spins = 0;
+      while (__hip_atomic_load(&bar[32], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1u << 27)) {
+          atomicOr(err, 1u);
+          break;
+        }
+      }
+    }
+    gen += 1;
+  }
+  asm volatile("" ::: "memory");
+  __syncthreads();
+}
+
 __device__ __forceinline__ u32 read_gen(const u32 *bar) {
   return __hip_atomic_load(&bar[32], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }

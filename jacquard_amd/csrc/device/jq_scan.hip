@@ -26,7 +26,7 @@
 
 namespace jqd {
 
-__device__ u32 g_barrier[kBarrierSlots * kSlotWords];
+__device__ u32 g_barrier[kCohSlots * kCohSlotWords];
 __device__ u32 g_error;
 
 namespace {
@@ -167,8 +167,8 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
     const uint4 ckf = ld4(script + p_ck + 4096 + 4 * t);
     u32 old_v = 0, old_x = 0;
     if ((u32)t < num_ios) {
-      old_v = in[io_offset + (u32)t];
-      if (is_x) old_x = in[xoff + io_offset + (u32)t];
+      old_v = ld_coherent(in + io_offset + (u32)t);
+      if (is_x) old_x = ld_coherent(in + xoff + io_offset + (u32)t);
     }
     u32 acc_v = 0, acc_x = 0;
     {
@@ -180,8 +180,8 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
         if (e[k].y) {
           const u32 *base = (e[k].x >> 31) ? out : in;
           u32 w = e[k].x & 0x7fffffffu;
-          v[k] = base[w];
-          if (is_x) x[k] = base[xoff + w];
+          v[k] = ld_coherent(base + w);
+          if (is_x) x[k] = ld_coherent(base + xoff + w);
         }
       }
 #pragma unroll
@@ -200,7 +200,7 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
       if (!er.y) continue;
       const u32 *base = (er.x >> 31) ? out : in;
       u32 w = er.x & 0x7fffffffu;
-      u32 v = base[w], x = is_x ? base[xoff + w] : 0u;
+      u32 v = ld_coherent(base + w), x = is_x ? ld_coherent(base + xoff + w) : 0u;
       u32 m = er.y;
       while (m) {
         u32 low = m & (0u - m);
@@ -391,15 +391,15 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
         u32 ck = (gather_bits(wo_v, ck0, ck1, ck2, ck3) & ~f.y) ^ f.x;
         u32 data = wo_v[t] ^ f.z;
         u32 w = io_offset + (u32)t;
-        out[w] = (old_v & ~ck) | (data & ck);
+        st_coherent(out + w, (old_v & ~ck) | (data & ck));
         if (is_x) {
           u32 ckx = gather_bits(wo_x, ck0, ck1, ck2, ck3) & ~f.y;
-          out[xoff + w] = (old_x & ~ck & ~ckx) | (wo_x[t] & ck & ~ckx) | ckx;
+          st_coherent(out + xoff + w, (old_x & ~ck & ~ckx) | (wo_x[t] & ck & ~ckx) | ckx);
         }
         if (TIMING) {
           u16 arr = wo_arr[t];
           if (A.arrival_state_offset != 0)
-            out[(u32)A.arrival_state_offset + w] = (u32)arr;
+            st_coherent(out + (u32)A.arrival_state_offset + w, (u32)arr);
           if (ck != 0 && A.constraints) {
             u32 cons = A.constraints[w];
             if (cons != 0) {
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
       u32 pb = PROFILE ? A.part_index[(size_t)m * A.num_blocks + b] : 0u;
       run_block<TIMING, PROFILE>(A, A.blocks_data + s0, in, out, c, pb, st_v,
                                  st_x, hin_v, hin_x, wo_v, wo_x, st_arr, wo_arr);
-      grid_sync(A.bar, A.num_blocks, gen, &g_error);
+      grid_sync_coherent(A.bar, A.num_blocks, gen, &g_error);
     }
   }
 }
@@ -481,7 +481,7 @@ int launch_scan(ScanArgs &A, hipStream_t stream) {
   u32 *bar_base = nullptr;
   if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(g_barrier)) != hipSuccess)
     return fail("hipGetSymbolAddress(g_barrier) failed");
-  A.bar = bar_base + kSlotWords * (__atomic_fetch_add(&g_slot, 1u, __ATOMIC_RELAXED) % kBarrierSlots);
+  A.bar = bar_base + kCohSlotWords * (__atomic_fetch_add(&g_slot, 1u, __ATOMIC_RELAXED) % kCohSlots);
   void *args[] = {&A};
   hipError_t e = hipLaunchCooperativeKernel(
       reinterpret_cast<const void *>(&scan_kernel<TIMING, PROFILE>),

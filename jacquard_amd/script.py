@@ -256,6 +256,30 @@ class FlattenedScriptV1:
                 tm += 2 * int(h[2])
         return 4 * (total_script + g + io + sr + tm)
 
+    def algorithmic_lds_bytes_per_cycle(self):
+        """LDS bytes of one lane-cycle in the interpreter's access pattern
+        (csrc/kernel_v1_impl.cuh): per boomerang stage the 8192 shuffle bit
+        gathers, the stage write, the hier0-3 reads/writes and the tail write
+        (880 words) and 2 words per writeout hook fired; per partition the
+        SRAM/dup and clock-enable permutations (2 x 8192), the writeout
+        read/writes (3 x 256) and the global-read store (256); doubled for
+        X-capable partitions.  Timing adds u16 arrival traffic of about 700
+        per stage plus one per writeout word."""
+        bd = self.blocks_data
+        total = 0
+        for off, _length in self.parts[:, 3:5]:
+            h = bd[int(off):int(off) + 256]
+            S = int(h[0])
+            x = 2 if (self.xprop_enabled and h[8]) else 1
+            hooks = h[128:256].astype(np.uint32)
+            nh = int(np.count_nonzero((hooks & 0xFFFF) != 0xFFFF)
+                     + np.count_nonzero((hooks >> 16) != 0xFFFF))
+            words = S * (8192 + 880) + 2 * nh + 2 * 8192 + 3 * 256 + 256
+            total += 4 * x * words
+            if self.timing_arrivals_enabled:
+                total += 2 * (S * 700 + int(h[2]))
+        return total
+
 
 def compile_design(netlist, num_blocks, level_split=0, xprop=False, **kw):
     """Convenience: partition + flatten with the heuristic cost."""
