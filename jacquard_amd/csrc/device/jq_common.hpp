@@ -101,8 +101,10 @@ __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" 
 constexpr int kCohSlots = 64;
 constexpr int kCohSlotWords = 4096;
 
-__device__ __forceinline__ void grid_sync_coherent(u32 *bar, u32 n, u32 &gen, u32 *err) {
-  wait_vmem();
+// The arrival and wait of grid_sync_coherent, for callers that have already
+// waited for their stores (wait_vmem) in every wave and may have loads in
+// flight that the barrier must not wait for.
+__device__ __forceinline__ void grid_arrive_wait(u32 *bar, u32 n, u32 &gen, u32 *err) {
   __syncthreads();
   if (threadIdx.x == 0) {
     u32 *set = bar + 256 + ((gen + 1u) & 1u) * 2048;
@@ -142,6 +144,11 @@ __device__ __forceinline__ void grid_sync_coherent(u32 *bar, u32 n, u32 &gen, u3
   }
   asm volatile("" ::: "memory");
   __syncthreads();
+}
+
+__device__ __forceinline__ void grid_sync_coherent(u32 *bar, u32 n, u32 &gen, u32 *err) {
+  wait_vmem();
+  grid_arrive_wait(bar, n, gen, err);
 }
 
 __device__ __forceinline__ u32 read_gen(const u32 *bar) {

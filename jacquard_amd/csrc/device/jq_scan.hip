@@ -104,72 +104,181 @@ __device__ __forceinline__ void push_event(EvHdr *ev, u32 cap, u32 type,
   }
 }
 
+// Header of one partition (src/flatten.rs:198-258) and this thread's
+// writeout hook; `base` is the header's word offset in blocks_data.
+struct PartHdr {
+  size_t base;
+  u32 num_stages, is_last, num_ios, io_offset, num_srams, sram_offset, rounds,
+      num_dups, xoff, hook;
+  bool is_x;
+};
+
+__device__ __forceinline__ PartHdr load_hdr(const u32 *__restrict__ bd, size_t off) {
+  PartHdr h;
+  const u32 *s = bd + off;
+  h.base = off;
+  h.num_stages = s[0];
+  h.is_last = s[1];
+  h.num_ios = s[2];
+  h.io_offset = s[3];
+  h.num_srams = s[4];
+  h.sram_offset = s[5];
+  h.rounds = s[6];
+  h.num_dups = s[7];
+  h.is_x = s[8] != 0;
+  h.xoff = s[9];
+  const u32 hook = s[128 + (threadIdx.x >> 1)];
+  h.hook = (threadIdx.x & 1) ? (hook >> 16) : (hook & 0xffffu);
+  return h;
+}
+
+__device__ __forceinline__ PartHdr no_hdr() {
+  PartHdr h{};
+  return h;  // num_stages == 0: nothing to run
+}
+
+// Words of one partition's script section (src/flatten.rs:873-1127).
+__device__ __forceinline__ size_t part_words(const PartHdr &h) {
+  return 256 + (size_t)512 * h.rounds + (size_t)5120 * (h.num_stages + 2);
+}
+
+// Everything a partition loads before its state gather: the global-read
+// entries (up to kPreRounds rounds), the stage-0 permutation and wave 0's
+// stage-0 tree flags, both writeout permutation sections and the commit's
+// old values.  Script words are read-only and the old values are the
+// partition's own writeouts of the previous cycle, so these loads are
+// issued before the barrier that precedes the partition, and their latency
+// hides behind the barrier wait (or the previous partition of the block).
+constexpr u32 kPreRounds = 16;
+struct PartPre {
+  uint2 e[kPreRounds];
+  uint4 q0, q1, q2, q3, fA, fB, fC, fD;
+  uint4 sd0, sd1, sd2, sd3, sdf, ck0, ck1, ck2, ck3, ckf;
+  u32 old_v, old_x;
+};
+
+__device__ __forceinline__ void load_pre(PartPre &P, const u32 *__restrict__ bd,
+                                         const PartHdr &h, const u32 *in) {
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const size_t pi = h.base + 256;
+  const uint2 *gr = reinterpret_cast<const uint2 *>(bd + pi);
+#pragma unroll
+  for (u32 k = 0; k < kPreRounds; ++k)
+    P.e[k] = (k < h.rounds) ? gr[256 * k + t] : make_uint2(0, 0);
+  const size_t p_st = pi + (size_t)512 * h.rounds;
+  const size_t p_sd = p_st + (size_t)5120 * h.num_stages;
+  const size_t p_ck = p_sd + 5120;
+  P.q0 = ld4(bd + p_st + 4 * t);
+  P.q1 = ld4(bd + p_st + 1024 + 4 * t);
+  P.q2 = ld4(bd + p_st + 2048 + 4 * t);
+  P.q3 = ld4(bd + p_st + 3072 + 4 * t);
+  // wave 0's tree flags of stage 0 (words 0..255 of the flag block)
+  P.fA = P.fB = P.fC = P.fD = make_uint4(0, 0, 0, 0);
+  if (t < 64) {
+    const u32 *fl0 = bd + p_st + 4096;
+    P.fA = ld4(fl0 + 4 * lane);
+    P.fB = ld4(fl0 + 4 * (64 + lane));
+    P.fC = ld4(fl0 + 4 * (128 + lane));
+    P.fD = ld4(fl0 + 4 * (192 + lane));
+  }
+  P.sd0 = ld4(bd + p_sd + 4 * t);
+  P.sd1 = ld4(bd + p_sd + 1024 + 4 * t);
+  P.sd2 = ld4(bd + p_sd + 2048 + 4 * t);
+  P.sd3 = ld4(bd + p_sd + 3072 + 4 * t);
+  P.sdf = ld4(bd + p_sd + 4096 + 4 * t);
+  P.ck0 = ld4(bd + p_ck + 4 * t);
+  P.ck1 = ld4(bd + p_ck + 1024 + 4 * t);
+  P.ck2 = ld4(bd + p_ck + 2048 + 4 * t);
+  P.ck3 = ld4(bd + p_ck + 3072 + 4 * t);
+  P.ckf = ld4(bd + p_ck + 4096 + 4 * t);
+  P.old_v = P.old_x = 0;
+  if ((u32)t < h.num_ios) {
+    P.old_v = ld_coherent(in + h.io_offset + (u32)t);
+    if (h.is_x) P.old_x = ld_coherent(in + h.xoff + h.io_offset + (u32)t);
+  }
+}
+
+// Value of lane L + HW on lane L (for the lanes where that stays inside
+// the wave's 64 lanes, rows of 16 for HW < 16): v_permlane32_swap moves the
+// upper half down, v_permlane16_swap odd rows to even rows, DPP row_shl
+// shifts within a row.
+template <int HW>
+__device__ __forceinline__ u32 from_lane_up(u32 v) {
+  if constexpr (HW == 32) {
+    return __builtin_amdgcn_permlane32_swap(v, v, false, false)[1];
+  } else if constexpr (HW == 16) {
+    return __builtin_amdgcn_permlane16_swap(v, v, false, false)[1];
+  } else {
+    return __builtin_amdgcn_update_dpp(0u, v, 0x100 + HW, 0xF, 0xF, false);
+  }
+}
+
+// One hier level of wave 0's tree (kernel_v1_impl.cuh:236-293): on entry
+// lane i holds word 2HW + i (lanes i < 2HW); on exit lane i < HW holds word
+// HW + i, also written to the layer.  Word w's flags are fA on lane w.
+template <int HW, bool TIMING>
+__device__ __forceinline__ void tree_level(u32 &cur, u32 &curx, u16 &cura, const uint4 &fA,
+                                           bool is_x, int lane, u32 *st_v, u32 *st_x,
+                                           u16 *st_arr) {
+  const u32 b = from_lane_up<HW>(cur);
+  const u32 xa = from_lane_up<HW>(fA.x), xb = from_lane_up<HW>(fA.y);
+  const u32 ob = from_lane_up<HW>(fA.z);
+  const u32 n = gate(cur, b, xa, xb, ob);
+  u32 nx = 0;
+  if (is_x) nx = gate_x(cur, curx, b, from_lane_up<HW>(curx), xa, xb, ob);
+  u16 na = 0;
+  if (TIMING) {
+    const u32 dly = from_lane_up<HW>(fA.w);
+    na = arr_step(cura, (u16)from_lane_up<HW>((u32)cura), ob, dly);
+  }
+  if (lane < HW) {
+    st_v[HW + lane] = n;
+    if (is_x) st_x[HW + lane] = nx;
+    if (TIMING) st_arr[HW + lane] = na;
+  }
+  cur = n;
+  curx = nx;
+  cura = na;
+}
+
+// One partition (src/sim/cpu_reference.rs:25-262 per partition;
+// kernel_v1_impl.cuh:62-558), its prologue loads already in P.
 template <bool TIMING, bool PROFILE>
-__device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
-                          const u32 *__restrict__ in, u32 *__restrict__ out,
-                          u32 cycle, u32 part_base, u32 *st_v, u32 *st_x,
-                          u32 *hin_v, u32 *hin_x, u32 *wo_v, u32 *wo_x,
-                          u16 *st_arr, u16 *wo_arr) {
+__device__ void run_partition(const ScanArgs &A, const u32 *__restrict__ bd,
+                              const PartHdr &h, PartPre &P,
+                              const u32 *__restrict__ in, u32 *__restrict__ out,
+                              u32 cycle, u32 part, u32 *st_v, u32 *st_x,
+                              u32 *hin_v, u32 *hin_x, u32 *wo_v, u32 *wo_x,
+                              u16 *st_arr, u16 *wo_arr) {
   const int t = threadIdx.x;
   const int lane = t & 63;
   const bool w0 = t < 64;
-  size_t pi = 0;
-  u32 part = part_base;
-  for (;;) {
-    const u32 num_stages = script[pi + 0];
-    if (num_stages == 0) break;
-    long long t_start = 0;
-    if (PROFILE) t_start = clock64();
-    const u32 is_last = script[pi + 1];
-    const u32 num_ios = script[pi + 2];
-    const u32 io_offset = script[pi + 3];
-    const u32 num_srams = script[pi + 4];
-    const u32 sram_offset = script[pi + 5];
-    const u32 rounds = script[pi + 6];
-    const u32 num_dups = script[pi + 7];
-    const bool is_x = script[pi + 8] != 0;
-    const u32 xoff = script[pi + 9];
-    u32 hook = script[pi + 128 + (t >> 1)];
-    hook = (t & 1) ? (hook >> 16) : (hook & 0xffffu);
-    pi += 256;
+  long long t_start = 0;
+  if (PROFILE) t_start = clock64();
+  const u32 num_stages = h.num_stages;
+  const u32 num_ios = h.num_ios;
+  const u32 io_offset = h.io_offset;
+  const u32 num_srams = h.num_srams;
+  const u32 sram_offset = h.sram_offset;
+  const u32 rounds = h.rounds;
+  const u32 num_dups = h.num_dups;
+  const bool is_x = h.is_x;
+  const u32 xoff = h.xoff;
+  const u32 hook = h.hook;
+  const u32 *script = bd + h.base;
+  size_t pi = 256;
+  const uint2 *gr = reinterpret_cast<const uint2 *>(script + pi);
+  const uint2 *e = P.e;
+  uint4 q0 = P.q0, q1 = P.q1, q2 = P.q2, q3 = P.q3;
+  uint4 fA = P.fA, fB = P.fB, fC = P.fC, fD = P.fD;
+  const uint4 sd0 = P.sd0, sd1 = P.sd1, sd2 = P.sd2, sd3 = P.sd3, sdf = P.sdf;
+  const uint4 ck0 = P.ck0, ck1 = P.ck1, ck2 = P.ck2, ck3 = P.ck3, ckf = P.ckf;
+  const u32 old_v = P.old_v, old_x = P.old_x;
+  const size_t p_st = pi + (size_t)512 * rounds;
 
-    // ---- global read (pext-gather), kernel_v1_impl.cuh:87-151.  Every
-    // load whose address is known from the header is issued here, before
-    // the first wait: the read entries (up to kPreRounds rounds), then the
-    // stage-0 permutation, both writeout permutation sections and the
-    // commit's old values, so one partition costs two dependent memory
-    // round trips before its first boomerang stage instead of six.
-    constexpr u32 kPreRounds = 16;
-    const uint2 *gr = reinterpret_cast<const uint2 *>(script + pi);
-    uint2 e[kPreRounds];
-#pragma unroll
-    for (u32 k = 0; k < kPreRounds; ++k)
-      e[k] = (k < rounds) ? gr[256 * k + t] : make_uint2(0, 0);
-    const size_t p_st = pi + (size_t)512 * rounds;
-    const size_t p_sd = p_st + (size_t)5120 * num_stages;
-    const size_t p_ck = p_sd + 5120;
-    uint4 q0 = ld4(script + p_st + 4 * t), q1 = ld4(script + p_st + 1024 + 4 * t);
-    uint4 q2 = ld4(script + p_st + 2048 + 4 * t), q3 = ld4(script + p_st + 3072 + 4 * t);
-    // wave 0's tree flags of stage 0 (words 0..255 of the flag block)
-    uint4 fA = make_uint4(0, 0, 0, 0), fB = fA, fC = fA, fD = fA;
-    if (w0) {
-      const u32 *fl0 = script + p_st + 4096;
-      fA = ld4(fl0 + 4 * lane);
-      fB = ld4(fl0 + 4 * (64 + lane));
-      fC = ld4(fl0 + 4 * (128 + lane));
-      fD = ld4(fl0 + 4 * (192 + lane));
-    }
-    const uint4 sd0 = ld4(script + p_sd + 4 * t), sd1 = ld4(script + p_sd + 1024 + 4 * t);
-    const uint4 sd2 = ld4(script + p_sd + 2048 + 4 * t), sd3 = ld4(script + p_sd + 3072 + 4 * t);
-    const uint4 sdf = ld4(script + p_sd + 4096 + 4 * t);
-    const uint4 ck0 = ld4(script + p_ck + 4 * t), ck1 = ld4(script + p_ck + 1024 + 4 * t);
-    const uint4 ck2 = ld4(script + p_ck + 2048 + 4 * t), ck3 = ld4(script + p_ck + 3072 + 4 * t);
-    const uint4 ckf = ld4(script + p_ck + 4096 + 4 * t);
-    u32 old_v = 0, old_x = 0;
-    if ((u32)t < num_ios) {
-      old_v = ld_coherent(in + io_offset + (u32)t);
-      if (is_x) old_x = ld_coherent(in + xoff + io_offset + (u32)t);
-    }
+    // ---- global read (pext-gather), kernel_v1_impl.cuh:87-151: the state
+    // words are gathered now, after the barrier that published them
     u32 acc_v = 0, acc_x = 0;
     {
       u32 v[kPreRounds], x[kPreRounds];
@@ -246,7 +355,7 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
         u32 o2 = gate(s0, s2, fC.x, fC.y, fC.z);
         u32 o3 = gate(s1, s3, fD.x, fD.y, fD.z);
         u32 o1 = gate(o2, o3, fB.x, fB.y, fB.z);
-        u32 o2x = 0, o3x = 0, o1x = 0, curx = 0;
+        u32 o2x = 0, o3x = 0, o1x = 0;
         if (is_x) {
           u32 x0 = hin_x[lane], x1 = hin_x[64 + lane];
           u32 x2 = hin_x[128 + lane], x3 = hin_x[192 + lane];
@@ -254,81 +363,58 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
           o3x = gate_x(s1, x1, s3, x3, fD.x, fD.y, fD.z);
           o1x = gate_x(o2, o2x, o3, o3x, fB.x, fB.y, fB.z);
         }
-        u16 a2 = 0, a3 = 0, a1 = 0, cura = 0;
+        u16 a2 = 0, a3 = 0, a1 = 0;
         if (TIMING) {
           a2 = arr_step(0, 0, fC.z, fC.w);
           a3 = arr_step(0, 0, fD.z, fD.w);
           a1 = arr_step(a2, a3, fB.z, fB.w);
         }
-        // hier2: lanes 32..63 -> word L from words L+32 (lane L-32) and
-        // L+64 (own hier1 word)
-        u32 cur = 0;
-        {
-          u32 a = __shfl(o1, (lane + 32) & 63);
-          u32 ax = is_x ? __shfl(o1x, (lane + 32) & 63) : 0u;
-          u32 aa = TIMING ? (u32)__shfl((int)a1, (lane + 32) & 63) : 0u;
-          if (lane >= 32) {
-            cur = gate(a, o1, fA.x, fA.y, fA.z);
-            if (is_x) curx = gate_x(a, ax, o1, o1x, fA.x, fA.y, fA.z);
-            if (TIMING) cura = arr_step((u16)aa, a1, fA.z, fA.w);
-          }
+        st_v[128 + lane] = o2;
+        st_v[192 + lane] = o3;
+        st_v[64 + lane] = o1;
+        if (is_x) {
+          st_x[128 + lane] = o2x;
+          st_x[192 + lane] = o3x;
+          st_x[64 + lane] = o1x;
         }
-        // hier3..7: lanes [hw,2hw) from lanes L+hw and L+2hw
-#pragma unroll
-        for (int hw = 16; hw >= 1; hw >>= 1) {
-          u32 a = __shfl(cur, (lane + hw) & 63);
-          u32 b = __shfl(cur, (lane + 2 * hw) & 63);
-          u32 ax = 0, bx = 0, aa = 0, ba = 0;
-          if (is_x) {
-            ax = __shfl(curx, (lane + hw) & 63);
-            bx = __shfl(curx, (lane + 2 * hw) & 63);
-          }
-          if (TIMING) {
-            aa = (u32)__shfl((int)cura, (lane + hw) & 63);
-            ba = (u32)__shfl((int)cura, (lane + 2 * hw) & 63);
-          }
-          if (lane >= hw && lane < 2 * hw) {
-            cur = gate(a, b, fA.x, fA.y, fA.z);
-            if (is_x) curx = gate_x(a, ax, b, bx, fA.x, fA.y, fA.z);
-            if (TIMING) cura = arr_step((u16)aa, (u16)ba, fA.z, fA.w);
-          }
+        if (TIMING) {
+          st_arr[128 + lane] = a2;
+          st_arr[192 + lane] = a3;
+          st_arr[64 + lane] = a1;
         }
-        // in-word tail hier8..12 on lane 0 (kernel_v1_impl.cuh:294-346)
-        u32 v1 = __shfl(cur, 1);
-        u32 v1x = is_x ? __shfl(curx, 1) : 0u;
+        // hier2..7: the node of word hw + i (hw = 32, 16, ..., 1) is
+        // computed on lane i from lanes i (word 2hw + i) and i + hw (word
+        // 3hw + i); values and the node's flags (word L's on lane L in fA)
+        // move up by hw with v_permlane32_swap, v_permlane16_swap and DPP
+        // row shifts, a few cycles each, instead of LDS permutes
+        u32 cur = o1, curx = o1x;
+        u16 cura = a1;
+        tree_level<32, TIMING>(cur, curx, cura, fA, is_x, lane, st_v, st_x, st_arr);
+        tree_level<16, TIMING>(cur, curx, cura, fA, is_x, lane, st_v, st_x, st_arr);
+        tree_level<8, TIMING>(cur, curx, cura, fA, is_x, lane, st_v, st_x, st_arr);
+        tree_level<4, TIMING>(cur, curx, cura, fA, is_x, lane, st_v, st_x, st_arr);
+        tree_level<2, TIMING>(cur, curx, cura, fA, is_x, lane, st_v, st_x, st_arr);
+        tree_level<1, TIMING>(cur, curx, cura, fA, is_x, lane, st_v, st_x, st_arr);
+        // in-word tail hier8..12 on lane 0 from word 1, now on lane 0
+        // (kernel_v1_impl.cuh:294-346)
         if (lane == 0) {
+          const u32 v1 = cur, v1x = curx;
           u32 xa = fA.x, xb = fA.y, ob = fA.z;
           u32 r8 = ((v1 << 16) ^ xa) & ((v1 ^ xb) | ob) & 0xffff0000u;
           u32 r9 = ((r8 >> 8) ^ xa) & (((r8 >> 16) ^ xb) | ob) & 0xff00u;
           u32 r10 = ((r9 >> 4) ^ xa) & (((r9 >> 8) ^ xb) | ob) & 0xf0u;
           u32 r11 = ((r10 >> 2) ^ xa) & (((r10 >> 4) ^ xb) | ob) & 0xcu;
           u32 r12 = ((r11 >> 1) ^ xa) & (((r11 >> 2) ^ xb) | ob) & 0x2u;
-          cur = r8 | r9 | r10 | r11 | r12;
+          st_v[0] = r8 | r9 | r10 | r11 | r12;
           if (is_x) {
             u32 x8 = gate_x(v1 << 16, v1x << 16, v1, v1x, xa, xb, ob) & 0xffff0000u;
             u32 x9 = gate_x(r8 >> 8, x8 >> 8, r8 >> 16, x8 >> 16, xa, xb, ob) & 0xff00u;
             u32 x10 = gate_x(r9 >> 4, x9 >> 4, r9 >> 8, x9 >> 8, xa, xb, ob) & 0xf0u;
             u32 x11 = gate_x(r10 >> 2, x10 >> 2, r10 >> 4, x10 >> 4, xa, xb, ob) & 0xcu;
             u32 x12 = gate_x(r11 >> 1, x11 >> 1, r11 >> 2, x11 >> 2, xa, xb, ob) & 0x2u;
-            curx = x8 | x9 | x10 | x11 | x12;
+            st_x[0] = x8 | x9 | x10 | x11 | x12;
           }
-          cura = 0;
-        }
-        st_v[128 + lane] = o2;
-        st_v[192 + lane] = o3;
-        st_v[64 + lane] = o1;
-        st_v[lane] = cur;
-        if (is_x) {
-          st_x[128 + lane] = o2x;
-          st_x[192 + lane] = o3x;
-          st_x[64 + lane] = o1x;
-          st_x[lane] = curx;
-        }
-        if (TIMING) {
-          st_arr[128 + lane] = a2;
-          st_arr[192 + lane] = a3;
-          st_arr[64 + lane] = a1;
-          st_arr[lane] = cura;
+          if (TIMING) st_arr[0] = 0;
         }
       }
       __syncthreads();
@@ -415,16 +501,12 @@ __device__ void run_block(const ScanArgs &A, const u32 *__restrict__ script,
         }
       }
     }
-    pi += 5120;
     if (PROFILE) {
       __syncthreads();
       if (t == 0 && cycle < A.sample_cycles && part < A.num_parts)
         A.samples[(size_t)cycle * A.num_parts + part] = (u32)(clock64() - t_start);
     }
-    ++part;
     __syncthreads();
-    if (is_last) break;
-  }
 }
 
 template <bool TIMING, bool PROFILE>
@@ -445,19 +527,49 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
   u32 gen = read_gen(A.bar);
   __syncthreads();
   const u32 b = blockIdx.x;
+  const u32 nb = A.num_blocks, M = A.num_major_stages;
+  const u32 *bd = A.blocks_data;
+  // the block's partitions in execution order: cycle, major stage, then
+  // position in the stage's section of blocks_data (kernel_v1_impl.cuh:
+  // 565-626); h / P always describe the next partition to run
+  PartHdr h = M ? load_hdr(bd, A.blocks_start[b]) : no_hdr();
+  PartPre P;
+  if (h.num_stages) load_pre(P, bd, h, A.states);
   for (u32 c = 0; c < A.num_cycles; ++c) {
     const u32 *in = A.states + (size_t)c * A.state_size;
     u32 *out = A.states + (size_t)(c + 1) * A.state_size;
-    for (u32 m = 0; m < A.num_major_stages; ++m) {
-      usize s0 = A.blocks_start[(size_t)m * A.num_blocks + b];
-      u32 pb = PROFILE ? A.part_index[(size_t)m * A.num_blocks + b] : 0u;
-      run_block<TIMING, PROFILE>(A, A.blocks_data + s0, in, out, c, pb, st_v,
-                                 st_x, hin_v, hin_x, wo_v, wo_x, st_arr, wo_arr);
-      grid_sync_coherent(A.bar, A.num_blocks, gen, &g_error);
+    for (u32 m = 0; m < M; ++m) {
+      u32 part = PROFILE ? A.part_index[(size_t)m * nb + b] : 0u;
+      const bool wrap = m + 1 == M;
+      const u32 nc = wrap ? c + 1 : c, nm = wrap ? 0u : m + 1;
+      for (;;) {
+        const bool last = h.num_stages == 0 || h.is_last != 0;
+        // the successor's header is requested before this partition runs
+        PartHdr hn = no_hdr();
+        if (!last)
+          hn = load_hdr(bd, h.base + part_words(h));
+        else if (nc < A.num_cycles)
+          hn = load_hdr(bd, A.blocks_start[(size_t)nm * nb + b]);
+        if (h.num_stages) {
+          run_partition<TIMING, PROFILE>(A, bd, h, P, in, out, c, part, st_v, st_x,
+                                         hin_v, hin_x, wo_v, wo_x, st_arr, wo_arr);
+          ++part;
+        }
+        if (last) {
+          // this wave's stores complete before the next stage's prologue
+          // loads are issued, so the barrier need not wait for those loads
+          wait_vmem();
+          if (hn.num_stages) load_pre(P, bd, hn, A.states + (size_t)nc * A.state_size);
+          h = hn;
+          break;
+        }
+        if (hn.num_stages) load_pre(P, bd, hn, in);
+        h = hn;
+      }
+      grid_arrive_wait(A.bar, nb, gen, &g_error);
     }
   }
 }
-
 thread_local std::string t_err;
 u32 g_slot = 0;
 
