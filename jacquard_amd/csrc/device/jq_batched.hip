@@ -37,7 +37,8 @@
 
 namespace jqd {
 
-__device__ u32 gb_barrier[kBarrierSlots * kSlotWords];
+constexpr int kBatchedSlots = 256;  // lane groups in flight
+__device__ u32 gb_barrier[kBatchedSlots * kCohSlotWords];
 __device__ u32 gb_error;
 
 namespace {
@@ -172,7 +173,7 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
 #pragma unroll
       for (int j = 0; j < 32; ++j) {
         u32 e = src[j], v = 0;
-        if (e != kNoSource) v = ((e >> 31) ? out : in)[e & 0x7fffffffu];
+        if (e != kNoSource) v = ld_coherent(((e >> 31) ? out : in) + (e & 0x7fffffffu));
         layer[sw(t + 256 * j)] = v;
       }
     }
@@ -294,9 +295,9 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
           data |= ((layer[sw(P + 64 + k)] >> l) & 1u) << k;
         }
         u32 *ram = ram_group + (size_t)l * A.sram_size + so + (size_t)s * kSramWords;
-        r = ram[ra];
-        u32 w0 = ram[wa];
-        ram[wa] = (w0 & ~en) | (data & en);
+        r = ld_coherent(ram + ra);
+        u32 w0 = ld_coherent(ram + wa);
+        st_coherent(ram + wa, (w0 & ~en) | (data & en));
       }
       // transpose the 32 read words of each SRAM into lane-words
       u32 mine = 0;
@@ -322,7 +323,7 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
         uint4 f = *reinterpret_cast<const uint4 *>(fl + 4 * (q >> 5));
         u32 ck = (WO[sw(perm_at(sec, q))] & ~fbit(f.y, q)) ^ fbit(f.x, q);
         u32 d = WO[sw(q)] ^ fbit(f.z, q);
-        o[q] = (i0[q] & ~ck) | (d & ck);
+        st_coherent(o + q, (ld_coherent(i0 + q) & ~ck) | (d & ck));
       }
     }
     __syncthreads();
@@ -337,7 +338,7 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
 __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
   extern __shared__ u32 lds[];
   const u32 g = blockIdx.x / A.W, b = blockIdx.x % A.W;
-  u32 *bar = A.bar + kSlotWords * g;
+  u32 *bar = A.bar + kCohSlotWords * g;
   u32 gen = read_gen(bar);
   const size_t snap = (size_t)A.rio * 32;
   u32 *st0 = A.states + (size_t)g * 2 * snap;
@@ -367,7 +368,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
     }
     if (b == 0 && c > 0 && c - 1 < A.obs_cap) {
       u32 *o = A.obs_out + ((size_t)g * A.obs_cap + (c - 1)) * A.n_obs;
-      for (u32 k = t; k < A.n_obs; k += 256) o[k] = in[A.obs_bits[k]];
+      for (u32 k = t; k < A.n_obs; k += 256) o[k] = ld_coherent(in + A.obs_bits[k]);
     }
     // lanes that completed their cycles hand their slot to the next lane
     const bool fin = my_lane != kNoLane && c >= my_end;
@@ -376,21 +377,22 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
       if (b == 0 && t < 32 && fin) {
         u32 nl = atomicAdd(A.queue, 1u);
         if (nl < A.n_lanes) {
-          slots[l] = nl;
+          st_coherent(slots + l, nl);
           A.lane_info[2 * nl] = g * 32 + l;
           A.lane_info[2 * nl + 1] = c;
         } else {
-          slots[l] = kNoLane;
+          st_coherent(slots + l, kNoLane);
         }
       }
-      grid_sync(bar, A.W, gen, &gb_error);
+      grid_sync_coherent(bar, A.W, b, gen, &gb_error);
       // reset the finished slots' lane bit: state of this cycle's input,
       // clock flags set, the lane's SRAM words zeroed
-      for (size_t w = par; w < snap; w += npar) in[w] &= ~finmask;
-      for (u32 k = par; k < A.n_flag; k += npar) in[A.flag_bits[k]] |= finmask;
+      for (size_t w = par; w < snap; w += npar) st_coherent(in + w, ld_coherent(in + w) & ~finmask);
+      for (u32 k = par; k < A.n_flag; k += npar)
+        st_coherent(in + A.flag_bits[k], ld_coherent(in + A.flag_bits[k]) | finmask);
       for (u32 m = finmask; m; m &= m - 1) {
         u32 *ram = ram_group + (size_t)(__ffs(m) - 1) * A.sram_size;
-        for (u32 w = par; w < A.sram_size; w += npar) ram[w] = 0;
+        for (u32 w = par; w < A.sram_size; w += npar) st_coherent(ram + w, 0u);
       }
       if (fin) {
         my_lane = __hip_atomic_load(&slots[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -401,7 +403,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
           my_thr = A.lane_thresh[my_lane];
         }
       }
-      grid_sync(bar, A.W, gen, &gb_error);
+      grid_sync_coherent(bar, A.W, b, gen, &gb_error);
     }
     if (!__syncthreads_or(my_lane != kNoLane)) break;
     // stimulus of snapshot c+1: data bit k toggles for the lane in slot l
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
       u32 tog = (u32)(m >> (32 * ((t >> 5) & 1)));
       if (k < A.n_data && l == 0) {
         u32 p = A.data_bits[k];
-        out[p] = in[p] ^ tog;
+        st_coherent(out + p, ld_coherent(in + p) ^ tog);
       }
     }
     for (u32 mj = 0; mj < A.num_major; ++mj) {
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
                       ? A.prof + (size_t)c * A.n_parts + A.gidx[k] : nullptr;
       run_block_lanes(A, A.bd + A.bs[k], A.gsrc + (size_t)A.gidx[k] * kLayerWords, in,
                       out, ram_group, lds, prof);
-      grid_sync(bar, A.W, gen, &gb_error);
+      grid_sync_coherent(bar, A.W, b, gen, &gb_error);
     }
   }
   if (b == 0 && t == 0) A.group_cycles[g] = c;
@@ -484,7 +486,7 @@ int batched_impl(usize num_groups, usize blocks_per_group, usize num_major_stage
                  u32 sample_cycles, void *stream) {
   set_error("");
   if (num_groups == 0 || blocks_per_group == 0) return 0;
-  if (num_groups > (usize)kBarrierSlots) {
+  if (num_groups > (usize)kBatchedSlots) {
     return set_error("too many lane groups");
   }
   if (!gather_table || !gather_start || !slots || !queue || !lane_info ||
@@ -534,9 +536,9 @@ int batched_impl(usize num_groups, usize blocks_per_group, usize num_major_stage
   if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(gb_barrier)) != hipSuccess) {
     return set_error("hipGetSymbolAddress(gb_barrier) failed");
   }
-  u32 first = __atomic_fetch_add(&b_slot, (u32)num_groups, __ATOMIC_RELAXED) % kBarrierSlots;
-  if (first + num_groups > (usize)kBarrierSlots) first = 0;
-  A.bar = bar_base + kSlotWords * first;
+  u32 first = __atomic_fetch_add(&b_slot, (u32)num_groups, __ATOMIC_RELAXED) % kBatchedSlots;
+  if (first + num_groups > (usize)kBatchedSlots) first = 0;
+  A.bar = bar_base + kCohSlotWords * first;
   void *args[] = {&A};
   // Under stream capture (a hipGraph of the run) the launch is an ordinary
   // kernel node: co-residency of all workgroups is established above from

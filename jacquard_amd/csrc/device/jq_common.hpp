@@ -12,12 +12,11 @@ namespace jqd {
 
 constexpr int kThreads = 256;      // words per 8192-bit boomerang layer
 constexpr u32 kSramWords = 8192;   // one AIGPDK SRAM (src/aigpdk.rs:22)
-constexpr int kBarrierSlots = 256; // concurrent persistent launches
 
-// Each kernel source owns its barrier slots (count, generation pairs) and a
-// sticky error word as file-scope device arrays.  Every launch takes its own
-// slot; generations only grow and counts return to zero after every
-// barrier, so no per-launch reset is needed.
+// Each kernel source owns its barrier slots and a sticky error word as
+// file-scope device arrays.  Every launch takes its own slot; generations
+// only grow and counters return to zero after every barrier, so no
+// per-launch reset is needed.
 
 // Host side: records the message returned by jq_last_error() for this
 // thread; returns -1 (or 0 for an empty message).
@@ -25,47 +24,6 @@ int set_error(const char *msg);
 
 // Host side: reads (and optionally clears) the batched kernels' error word.
 int batched_error_flag(u32 *out, int clear);
-
-// Grid-wide barrier among `n` co-resident workgroups.  A slot is kSlotWords
-// u32 (256 bytes): arrival count at [0], generation at [32], on separate
-// 128-byte lines, so waiters polling the generation do not contend with the
-// arrivals' atomics, and concurrent barriers never share a line.  One
-// agent-scope release fence publishes the workgroup's stores (ordered after
-// them by the workgroup barrier), the arrival and the polling are relaxed
-// agent-scope atomics (an acquire on every poll would invalidate the XCD's
-// L2 once per poll, for every waiting workgroup), and one acquire fence
-// after the wait makes the other workgroups' stores visible.  The wait is
-// bounded: after ~2^27 polls it raises `err` and returns so the grid always
-// drains.  (Polling the arrival counter itself instead of a generation word
-// measured slower: the polls contend with the atomics on one line.)
-constexpr int kSlotWords = 64;
-
-__device__ __forceinline__ void grid_sync(u32 *bar, u32 n, u32 &gen, u32 *err) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    u32 arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == n - 1) {
-      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      // release: the counter reset is visible before the new generation
-      __hip_atomic_fetch_add(&bar[32], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      u32 spins = 0;
-      while (__hip_atomic_load(&bar[32], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins > (1u << 27)) {
-          atomicOr(err, 1u);
-          break;
-        }
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    gen += 1;
-  }
-  __syncthreads();
-}
 
 // Agent-coherent access to memory that other workgroups of the launch read
 // or write (relaxed agent-scope atomics: on gfx950 loads and stores with the
@@ -79,39 +37,49 @@ __device__ __forceinline__ void st_coherent(u32 *p, u32 v) {
 // Wait until this wave's memory operations have completed.
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// Grid barrier for kernels whose inter-workgroup data moves only through
-// ld_coherent / st_coherent.  Those accesses are coherent at agent scope by
-// themselves, so the barrier needs ordering, not cache maintenance: every
-// wave waits for its stores to complete, then the workgroup arrives; after
-// the generation changes, the workgroup's coherent loads are issued and see
-// every store made before any arrival.  This drops the agent-scope release
-// fence (an L2 writeback) and acquire fence (an L2 invalidate) that
-// grid_sync issues per workgroup, which serialise in each XCD's L2.
+// Grid-wide barrier among `n` co-resident workgroups whose inter-workgroup
+// data moves only through ld_coherent / st_coherent.  Those accesses are
+// coherent at agent scope by themselves, so the barrier needs ordering, not
+// cache maintenance: every wave waits for its stores to complete, then the
+// workgroup arrives; after the generation changes, the workgroup's coherent
+// loads are issued and see every store made before any arrival.  An
+// agent-scope release / acquire fence pair per workgroup (an L2 writeback
+// and an L2 invalidate on gfx950) serialises in each XCD's L2 and cost about
+// 2 us per barrier at 156 workgroups.
+//
+// Arrivals and polling are relaxed agent-scope atomics: an acquire on every
+// poll would invalidate the XCD's L2 once per poll.  Waiters poll a
+// generation word on its own 128-byte line, apart from the counters' lines.
+// The wait is bounded: after ~2^27 polls it raises `err` and returns, so the
+// grid always drains.
 //
 // A slot is kCohSlotWords u32: the generation at [32] and two counter sets
 // at 256 and 2304, used by alternate barriers, so the last arrivers' resets
 // need no wait (a set is reused two barriers later, after the resetting
 // workgroup's next arrival has waited for its stores).  A set holds a top
-// counter at [0] and, for n >= 128, eight per-XCD sub-counters at [128 *
-// (1 + x)] (workgroups are dispatched round-robin over the 8 XCDs, x =
-// blockIdx % 8): one contended atomic per arrival costs ~10 ns at the
-// coherence point, so 156 flat arrivals take ~1.5 us, two levels ~0.3 us.
-// Measured empty major stage at 156 workgroups: 5.2 us with grid_sync, 2.1 us
-// with this barrier.  Bounded wait as in grid_sync.
+// counter at [0] and, for n >= 128, eight sub-counters at [128 * (1 + x)],
+// x = member % 8 (per XCD when members are consecutive workgroups, which
+// are dispatched round-robin over the 8 XCDs): one contended atomic per
+// arrival costs ~10 ns at the coherence point, so 156 flat arrivals take
+// ~1.5 us, two levels ~0.3 us.  Measured empty major stage at 156
+// workgroups: 5.2 us with the fenced single-counter barrier, 2.0 us with
+// this one.
 constexpr int kCohSlots = 64;
 constexpr int kCohSlotWords = 4096;
 
 // The arrival and wait of grid_sync_coherent, for callers that have already
 // waited for their stores (wait_vmem) in every wave and may have loads in
-// flight that the barrier must not wait for.
-__device__ __forceinline__ void grid_arrive_wait(u32 *bar, u32 n, u32 &gen, u32 *err) {
+// flight that the barrier must not wait for.  `member` is the workgroup's
+// index among the n (0..n-1).
+__device__ __forceinline__ void grid_arrive_wait(u32 *bar, u32 n, u32 member, u32 &gen,
+                                                 u32 *err) {
   __syncthreads();
   if (threadIdx.x == 0) {
     u32 *set = bar + 256 + ((gen + 1u) & 1u) * 2048;
     bool last = false;
     if (n >= 128) {
-      const u32 x = blockIdx.x & 7u;
-      const u32 nx = (n - x + 7u) >> 3;  // workgroups with blockIdx % 8 == x
+      const u32 x = member & 7u;
+      const u32 nx = (n - x + 7u) >> 3;  // members with index % 8 == x
       u32 a = __hip_atomic_fetch_add(&set[128 * (1 + x)], 1u, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
       if (a == nx - 1u) {
@@ -146,9 +114,10 @@ __device__ __forceinline__ void grid_arrive_wait(u32 *bar, u32 n, u32 &gen, u32 
   __syncthreads();
 }
 
-__device__ __forceinline__ void grid_sync_coherent(u32 *bar, u32 n, u32 &gen, u32 *err) {
+__device__ __forceinline__ void grid_sync_coherent(u32 *bar, u32 n, u32 member, u32 &gen,
+                                                   u32 *err) {
   wait_vmem();
-  grid_arrive_wait(bar, n, gen, err);
+  grid_arrive_wait(bar, n, member, gen, err);
 }
 
 __device__ __forceinline__ u32 read_gen(const u32 *bar) {

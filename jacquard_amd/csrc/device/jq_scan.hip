@@ -566,7 +566,7 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
         if (hn.num_stages) load_pre(P, bd, hn, in);
         h = hn;
       }
-      grid_arrive_wait(A.bar, nb, gen, &g_error);
+      grid_arrive_wait(A.bar, nb, b, gen, &g_error);
     }
   }
 }
