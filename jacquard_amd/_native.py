@@ -76,6 +76,15 @@ def host():
         "jqh_script_sram_offsets": (c_vp, [c_vp]),
         "jqh_script_dff_data_pos": (c_vp, [c_vp]),
         "jqh_inject_timing": (ctypes.c_int, [c_vp, c_vp]),
+        "jqh_vcd_parse_body": (c_vp, [c_cp, c_u64, c_cp, c_vp, c_u32, c_vp, c_vp, c_vp,
+                                      c_vp, c_vp, c_u32, c_u32, c_u64]),
+        "jqh_vcd_num_words": (c_u64, [c_vp]),
+        "jqh_vcd_states": (c_vp, [c_vp]),
+        "jqh_vcd_num_offsets": (c_u64, [c_vp]),
+        "jqh_vcd_offsets": (c_vp, [c_vp]),
+        "jqh_vcd_timestamps": (c_vp, [c_vp]),
+        "jqh_vcd_free": (None, [c_vp]),
+        "jqh_vcd_last_error": (c_cp, []),
         "jqh_constraint_buffer": (None, [c_vp, c_u32, ctypes.c_uint16,
                                          ctypes.c_uint16, c_vp]),
         "jqh_validate": (ctypes.c_int, [c_vp, c_u64, c_vp, c_u64, c_u64,

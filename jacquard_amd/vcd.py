@@ -17,6 +17,8 @@ import re
 
 import numpy as np
 
+from . import _native as N
+
 U64_MAX = (1 << 64) - 1
 _PS_DIVISOR = {"s": 1, "ms": 10**3, "us": 10**6, "ns": 10**9, "ps": 10**12, "fs": 10**15}
 
@@ -165,7 +167,16 @@ def auto_scope(header, input_names):
 
 # ---------------------------------------------------------------- reader
 def parse_input_vcd(text, script, scope=None, max_clock_edges=None):
-    """(input_states uint32[(C+1)*rio], offsets_timestamps [(offset, ts)])."""
+    """(input_states uint32[(C+1)*rio], offsets_timestamps [(offset, ts)],
+    header)."""
+    header, body, code_map, flag_bits, rio = input_plan(text, script, scope)
+    states, offs = parse_body(body, code_map, flag_bits, rio, max_clock_edges)
+    return states, offs, header
+
+
+def input_plan(text, script, scope=None):
+    """Header, value-change text, and the map from VCD codes to the state
+    bits of the script's inputs: (header, body, code_map, flag_bits, rio)."""
     netlist = script.netlist
     ins, _ = netlist.ports()
     in_names = [n for (n, _, _) in ins]
@@ -174,7 +185,6 @@ def parse_input_vcd(text, script, scope=None, max_clock_edges=None):
     vcd2inp = match_vcd_inputs(sc, in_names)
     name2port = {n: (pin, flag) for (n, pin, flag) in ins}
     rio = script.reg_io_state_size
-    state = np.zeros(rio, np.uint32)
     # (code, pos) -> (state bit, flag bit or None)
     code_map = {}
     for (code, pos), name in vcd2inp.items():
@@ -183,80 +193,53 @@ def parse_input_vcd(text, script, scope=None, max_clock_edges=None):
         fbit = script.input_map[flag] if flag else None
         code_map.setdefault(code, {})[pos] = (bit, fbit)
     flag_bits = [script.input_map[f] for (_, _, f) in ins if f]
+    return header, body, code_map, flag_bits, rio
 
-    def getb(p):
-        return int(state[p >> 5] >> np.uint32(p & 31)) & 1
 
-    def setb(p, v):
-        if v:
-            state[p >> 5] |= np.uint32(1) << np.uint32(p & 31)
+def parse_body(body, code_map, flag_bits, rio, max_clock_edges=None):
+    """Value-change section -> (snapshots uint32[(C+1)*rio], offsets
+    [(word offset, previous active time)]), by the native streaming reader
+    (csrc/host/vcd.cpp; src/sim/vcd_io.rs:347-455).  code_map: {code: {bit
+    position in the value: (state bit, posedge flag bit or None)}}."""
+    import ctypes
+    L = N.host()
+    codes = list(code_map)
+    enc = [c.encode() for c in codes]
+    code_off = np.zeros(len(codes) + 1, np.uint32)
+    code_off[1:] = np.cumsum([len(e) for e in enc])
+    blob = b"".join(enc) or b"\0"
+    ent_start = np.zeros(len(codes) + 1, np.uint32)
+    pos, bit, flag = [], [], []
+    for i, c in enumerate(codes):
+        for p_, (b_, f_) in sorted(code_map[c].items()):
+            pos.append(p_)
+            bit.append(b_)
+            flag.append(0xFFFFFFFF if f_ is None else f_)
+        ent_start[i + 1] = len(pos)
+    arr = [np.ascontiguousarray(x, np.uint32) for x in (pos, bit, flag, flag_bits)]
+    data = body.encode() if isinstance(body, str) else bytes(body)
+    h = L.jqh_vcd_parse_body(data, len(data), blob, code_off.ctypes.data, len(codes),
+                             ent_start.ctypes.data, arr[0].ctypes.data, arr[1].ctypes.data,
+                             arr[2].ctypes.data, arr[3].ctypes.data, len(flag_bits), rio,
+                             int(max_clock_edges or 0))
+    if not h:
+        raise ValueError(L.jqh_vcd_last_error().decode())
+    try:
+        nw = int(L.jqh_vcd_num_words(h))
+        no = int(L.jqh_vcd_num_offsets(h))
+        states = np.ctypeslib.as_array(
+            ctypes.cast(L.jqh_vcd_states(h), ctypes.POINTER(ctypes.c_uint32)), (nw,)).copy()
+        if no:
+            off = np.ctypeslib.as_array(
+                ctypes.cast(L.jqh_vcd_offsets(h), ctypes.POINTER(ctypes.c_uint64)), (no,))
+            ts = np.ctypeslib.as_array(
+                ctypes.cast(L.jqh_vcd_timestamps(h), ctypes.POINTER(ctypes.c_uint64)), (no,))
+            offs = list(zip(off.tolist(), ts.tolist()))
         else:
-            state[p >> 5] &= ~(np.uint32(1) << np.uint32(p & 31))
-
-    snaps, offs = [], []
-    vcd_time_last_active = U64_MAX
-    vcd_time = 0
-    last_active = True
-    delayed = []
-    toks = body.split()
-    i = 0
-    while i < len(toks):
-        tok = toks[i]
-        i += 1
-        c0 = tok[0]
-        if c0 == "#":
-            t = int(tok[1:])
-            if t == vcd_time:
-                continue
-            if last_active:
-                snaps.append(state.copy())
-                offs.append((len(snaps) * rio, vcd_time_last_active))
-                for fb in flag_bits:
-                    setb(fb, 0)
-                if max_clock_edges is not None and len(offs) >= max_clock_edges:
-                    break
-                vcd_time_last_active = vcd_time
-            vcd_time = t
-            last_active = False
-            for p in dict.fromkeys(delayed):
-                state[p >> 5] ^= np.uint32(1) << np.uint32(p & 31)
-            delayed = []
-            continue
-        if c0 == "$":
-            if tok == "$comment":
-                while i < len(toks) and toks[i] != "$end":
-                    i += 1
-                i += 1
-            continue  # $dumpvars / $dumpall / $end markers
-        if c0 in "bBrR":
-            # vector or real value: the identifier is the next token
-            bits, code = tok[1:], toks[i]
-            i += 1
-            if c0 in "rR":
-                continue
-        else:
-            bits, code = tok[0], tok[1:]
-        m = code_map.get(code)
-        if not m:
-            continue
-        for pos, b in enumerate(bits):
-            e = m.get(pos)
-            if e is None:
-                continue
-            bit, fbit = e
-            old = getb(bit)
-            new = 1 if b == "1" else 0
-            if old == new:
-                continue
-            if fbit is not None:
-                if old == 0:
-                    last_active = True
-                    setb(fbit, 1)
-                delayed.append(bit)
-            else:
-                state[bit >> 5] ^= np.uint32(1) << np.uint32(bit & 31)
-    snaps.append(state.copy())
-    return np.concatenate(snaps), offs, header
+            offs = []
+    finally:
+        L.jqh_vcd_free(h)
+    return states, offs
 
 
 # ---------------------------------------------------------------- writers

@@ -1,0 +1,95 @@
+"""Pure-Python restatement of the input-VCD value-change loop.
+
+TEST INFRASTRUCTURE ONLY: the checker for the native streaming reader
+(jacquard_amd/csrc/host/vcd.cpp), imported by tests, never by the product
+package.
+
+Follows parse_input_vcd's body loop (src/sim/vcd_io.rs:347-455): a snapshot
+is pushed at the first new timestamp after one at which a clock edge
+happened (and once at the end), with the offset entry (words so far, time of
+the previous active timestamp; u64 max first); posedge flags are cleared
+after each pushed snapshot; data pins change at once, a clock pin's change
+sets its posedge flag on 0 -> 1 and is applied at the next new timestamp.
+Pinned end to end by the reference's golden output VCDs
+(tests/timing_test/gem_output_fixed.vcd, tests/assert_test/
+assert_test_output.vcd), which the product reproduces byte for byte.
+"""
+import numpy as np
+
+U64_MAX = (1 << 64) - 1
+
+
+def parse_body(body, code_map, flag_bits, rio, max_clock_edges=None):
+    """Same contract as jacquard_amd.vcd.parse_body."""
+    state = [0] * rio
+
+    def get(b):
+        return (state[b >> 5] >> (b & 31)) & 1
+
+    def flip(b):
+        state[b >> 5] ^= 1 << (b & 31)
+
+    snaps, offs = [], []
+    last_time = U64_MAX
+    now = 0
+    active = True
+    pending = []
+    toks = body.split()
+    i = 0
+    while i < len(toks):
+        tok = toks[i]
+        i += 1
+        if tok[0] == "#":
+            t = int(tok[1:])
+            if t == now:
+                continue
+            if active:
+                snaps.append(list(state))
+                offs.append((len(snaps) * rio, last_time))
+                for f in flag_bits:
+                    state[f >> 5] &= ~(1 << (f & 31))
+                if max_clock_edges and len(offs) >= max_clock_edges:
+                    break
+                last_time = now
+            now = t
+            active = False
+            for b in dict.fromkeys(pending):
+                flip(b)
+            pending = []
+            continue
+        if tok[0] == "$":
+            if tok == "$comment":
+                while i < len(toks) and toks[i] != "$end":
+                    i += 1
+                i += 1
+            continue
+        if tok[0] in "bBrR":
+            if i >= len(toks):
+                break
+            value, code = tok[1:], toks[i]
+            i += 1
+            if tok[0] in "rR":
+                continue
+        else:
+            value, code = tok[0], tok[1:]
+        targets = code_map.get(code)
+        if not targets:
+            continue
+        for pos, ch in enumerate(value):
+            tgt = targets.get(pos)
+            if tgt is None:
+                continue
+            bit, flag = tgt
+            new = 1 if ch == "1" else 0
+            old = get(bit)
+            if old == new:
+                continue
+            if flag is None:
+                flip(bit)
+                continue
+            if old == 0:
+                active = True
+                state[flag >> 5] |= 1 << (flag & 31)
+            pending.append(bit)
+    snaps.append(list(state))
+    return np.array(snaps, np.uint64).astype(np.uint32).reshape(-1), offs
