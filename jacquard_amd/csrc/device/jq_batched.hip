@@ -137,10 +137,15 @@ __device__ __forceinline__ void sec_fetch(const u32 *g, u32 *l) {
 // the workgroup barrier that precedes the first read of a fetched section.
 __device__ __forceinline__ void sec_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// Runs the partitions of one (major stage, block).  src holds the first
+// partition's gather-table entries on entry (when its script is not empty);
+// on return it holds those of the first partition of the block's next major
+// stage (script nscript, table ngsrc), loaded while this stage ran.
 __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__restrict__ script,
                                 const u32 *__restrict__ gsrc,
                                 const u32 *__restrict__ in, u32 *__restrict__ out,
-                                u32 *ram_group, u32 *lds, u32 *prof) {
+                                u32 *ram_group, u32 *lds, u32 *prof, u32 (&src)[32],
+                                const u32 *nscript, const u32 *ngsrc) {
   const u32 t = threadIdx.x;
   const u32 lane = t & 63;
   u32 *layer = lds, *WO = lds + kLayerWords;
@@ -149,7 +154,13 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
   size_t pi = 0;
   for (;; gsrc += kLayerWords) {
     const u32 S = script[pi];
-    if (S == 0) break;
+    if (S == 0) {  // no partition in this stage: fetch the next stage's entries
+      if (nscript[0] != 0) {
+#pragma unroll
+        for (int j = 0; j < 32; ++j) src[j] = ngsrc[t + 256 * j];
+      }
+      break;
+    }
     const unsigned long long tick0 = prof ? clock64() : 0;
     const u32 is_last = script[pi + 1];
     const u32 num_ios = script[pi + 2];
@@ -163,19 +174,29 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
     // sections: stage 0..S-1, SRAM/dup, clock enable, contiguous
     const u32 *secg = script + pi + 256 + (size_t)512 * R;
     pi += 256 + (size_t)512 * R + (size_t)kSecWords * (S + 2);
+    // the successor's first header word: the next partition here, or the
+    // next stage's first
+    const u32 next_s = is_last ? nscript[0] : script[pi];
 
     sec_fetch(secg, secb(0));
-    // ---- global read through the gather table: 32 independent loads
+    // ---- global read through the gather table: 32 independent loads of
+    // state words at entries prefetched one partition ahead
     {
-      u32 src[32];
-#pragma unroll
-      for (int j = 0; j < 32; ++j) src[j] = gsrc[t + 256 * j];
+      u32 v[32];
 #pragma unroll
       for (int j = 0; j < 32; ++j) {
-        u32 e = src[j], v = 0;
-        if (e != kNoSource) v = ld_coherent(((e >> 31) ? out : in) + (e & 0x7fffffffu));
-        layer[sw(t + 256 * j)] = v;
+        u32 e = src[j];
+        v[j] = e != kNoSource ? ld_coherent(((e >> 31) ? out : in) + (e & 0x7fffffffu)) : 0u;
       }
+      // the successor's entries load while this partition runs (the table
+      // is read-only)
+      if (next_s != 0) {
+        const u32 *nx = is_last ? ngsrc : gsrc + kLayerWords;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) src[j] = nx[t + 256 * j];
+      }
+#pragma unroll
+      for (int j = 0; j < 32; ++j) layer[sw(t + 256 * j)] = v[j];
     }
     sec_wait();
     __syncthreads();
@@ -354,6 +375,13 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
     my_seed = A.lane_seed[my_lane];
     my_thr = A.lane_thresh[my_lane];
   }
+  // gather entries of the block's first partition of major stage 0
+  u32 src[32];
+  if (A.bd[A.bs[b]] != 0) {
+    const u32 *g0 = A.gsrc + (size_t)A.gidx[b] * kLayerWords;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) src[j] = g0[t + 256 * j];
+  }
   __syncthreads();
   u32 c = 0;
   for (;; ++c) {
@@ -427,8 +455,12 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
       const size_t k = (size_t)mj * A.W + b;
       u32 *prof = (A.prof && g == 0 && c < A.prof_cycles)
                       ? A.prof + (size_t)c * A.n_parts + A.gidx[k] : nullptr;
+      // next major stage of this block (stage 0 of the next cycle after
+      // the last)
+      const size_t kn = mj + 1 < A.num_major ? k + A.W : (size_t)b;
       run_block_lanes(A, A.bd + A.bs[k], A.gsrc + (size_t)A.gidx[k] * kLayerWords, in,
-                      out, ram_group, lds, prof);
+                      out, ram_group, lds, prof, src, A.bd + A.bs[kn],
+                      A.gsrc + (size_t)A.gidx[kn] * kLayerWords);
       grid_sync_coherent(bar, A.W, b, gen, &gb_error);
     }
   }
