@@ -137,15 +137,37 @@ __device__ __forceinline__ void sec_fetch(const u32 *g, u32 *l) {
 // the workgroup barrier that precedes the first read of a fetched section.
 __device__ __forceinline__ void sec_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-// Runs the partitions of one (major stage, block).  src holds the first
-// partition's gather-table entries on entry (when its script is not empty);
-// on return it holds those of the first partition of the block's next major
-// stage (script nscript, table ngsrc), loaded while this stage ran.
+// Header of one partition (src/flatten.rs:198-258) and this thread's
+// writeout hook.
+struct BHdr {
+  u32 S, is_last, num_ios, io, ns, so, R, nd, hook;
+};
+
+__device__ __forceinline__ BHdr load_bhdr(const u32 *p) {
+  BHdr h;
+  h.S = p[0];
+  h.is_last = p[1];
+  h.num_ios = p[2];
+  h.io = p[3];
+  h.ns = p[4];
+  h.so = p[5];
+  h.R = p[6];
+  h.nd = p[7];
+  const u32 hk = p[128 + (threadIdx.x >> 1)];
+  h.hook = (threadIdx.x & 1) ? (hk >> 16) : (hk & 0xffffu);
+  return h;
+}
+
+// Runs the partitions of one (major stage, block).  On entry h and src hold
+// the first partition's header and gather-table entries (src only when the
+// script is not empty); on return they hold those of the first partition of
+// the block's next major stage (script nscript, table ngsrc), loaded while
+// this stage ran.
 __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__restrict__ script,
                                 const u32 *__restrict__ gsrc,
                                 const u32 *__restrict__ in, u32 *__restrict__ out,
-                                u32 *ram_group, u32 *lds, u32 *prof, u32 (&src)[32],
-                                const u32 *nscript, const u32 *ngsrc) {
+                                u32 *ram_group, u32 *lds, u32 *prof, BHdr &h,
+                                u32 (&src)[32], const u32 *nscript, const u32 *ngsrc) {
   const u32 t = threadIdx.x;
   const u32 lane = t & 63;
   u32 *layer = lds, *WO = lds + kLayerWords;
@@ -153,30 +175,24 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
   auto secb = [lds](u32 k) { return lds + 2 * kLayerWords + (k & 1u) * kSecWords; };
   size_t pi = 0;
   for (;; gsrc += kLayerWords) {
-    const u32 S = script[pi];
-    if (S == 0) {  // no partition in this stage: fetch the next stage's entries
-      if (nscript[0] != 0) {
+    if (h.S == 0) {  // no partition in this stage: fetch the next stage's first
+      h = load_bhdr(nscript);
+      if (h.S != 0) {
 #pragma unroll
         for (int j = 0; j < 32; ++j) src[j] = ngsrc[t + 256 * j];
       }
       break;
     }
     const unsigned long long tick0 = prof ? clock64() : 0;
-    const u32 is_last = script[pi + 1];
-    const u32 num_ios = script[pi + 2];
-    const u32 io = script[pi + 3];
-    const u32 ns = script[pi + 4];
-    const u32 so = script[pi + 5];
-    const u32 R = script[pi + 6];
-    const u32 nd = script[pi + 7];
-    u32 hook = script[pi + 128 + (t >> 1)];
-    hook = (t & 1) ? (hook >> 16) : (hook & 0xffffu);
+    const u32 S = h.S, is_last = h.is_last, num_ios = h.num_ios, io = h.io;
+    const u32 ns = h.ns, so = h.so, R = h.R, nd = h.nd, hook = h.hook;
     // sections: stage 0..S-1, SRAM/dup, clock enable, contiguous
     const u32 *secg = script + pi + 256 + (size_t)512 * R;
     pi += 256 + (size_t)512 * R + (size_t)kSecWords * (S + 2);
-    // the successor's first header word: the next partition here, or the
-    // next stage's first
-    const u32 next_s = is_last ? nscript[0] : script[pi];
+    // the successor's header (the next partition here, or the next stage's
+    // first) is requested now, a partition ahead of its use
+    h = is_last ? load_bhdr(nscript) : load_bhdr(script + pi);
+    const u32 next_s = h.S;
 
     sec_fetch(secg, secb(0));
     // ---- global read through the gather table: 32 independent loads of
@@ -375,9 +391,11 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
     my_seed = A.lane_seed[my_lane];
     my_thr = A.lane_thresh[my_lane];
   }
-  // gather entries of the block's first partition of major stage 0
+  // header and gather entries of the block's first partition of major
+  // stage 0
   u32 src[32];
-  if (A.bd[A.bs[b]] != 0) {
+  BHdr h = load_bhdr(A.bd + A.bs[b]);
+  if (h.S != 0) {
     const u32 *g0 = A.gsrc + (size_t)A.gidx[b] * kLayerWords;
 #pragma unroll
     for (int j = 0; j < 32; ++j) src[j] = g0[t + 256 * j];
@@ -459,7 +477,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
       // the last)
       const size_t kn = mj + 1 < A.num_major ? k + A.W : (size_t)b;
       run_block_lanes(A, A.bd + A.bs[k], A.gsrc + (size_t)A.gidx[k] * kLayerWords, in,
-                      out, ram_group, lds, prof, src, A.bd + A.bs[kn],
+                      out, ram_group, lds, prof, h, src, A.bd + A.bs[kn],
                       A.gsrc + (size_t)A.gidx[kn] * kLayerWords);
       grid_sync_coherent(bar, A.W, b, gen, &gb_error);
     }
