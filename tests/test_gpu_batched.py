@@ -95,3 +95,14 @@ def test_batched_graph_replays():
         got = run.lane_outputs()
         for lane in range(50):
             assert np.array_equal(got[lane], eager[lane]), (r, lane)
+
+
+def test_batched_wide_group_hierarchical_barrier():
+    """128 blocks per group: the group barrier counts arrivals through its
+    eight sub-counters (the path taken at 128 members or more)."""
+    nl, comp = design("c1", level_split=2, **CONFIG1)
+    s = comp.flatten(128)
+    seeds, probs, cycles = _lanes(40, 10, 60, 7)
+    run = BatchedRun(s, seeds, probs, cycles, groups=1)
+    print("blocks", s.num_blocks, "major", s.num_major_stages, flush=True)
+    _check(s, run, seeds, probs, cycles, range(40))

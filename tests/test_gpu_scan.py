@@ -31,6 +31,25 @@ def test_config1_2state(num_blocks, split):
     direct_check(s, got, cycles, 11, 0.5)
 
 
+@pytest.mark.parametrize("num_blocks", [128, 200])
+def test_many_blocks_hierarchical_barrier(num_blocks):
+    """At 128 workgroups or more the grid barrier counts arrivals through
+    eight sub-counters; most blocks here are empty and still take part."""
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    nl = Netlist.random(seed=5, gates=20000, depth=40, dffs=1024, inputs=64, outputs=32,
+                        geometric_p=0.3, invert_prob=0.5)
+    # small partitions: about ten per major stage, spread over the blocks
+    s = Compiled(nl, level_split=4, max_part_nodes=500).flatten(num_blocks)
+    cycles = 150
+    _say("blocks", num_blocks, "major", s.num_major_stages, "parts", s.num_parts)
+    st = stimulus.build_states(s, cycles, seed=21, p=0.3)
+    ref = st.copy()
+    oracle.run(s, ref, cycles)
+    got = sim.run_scan(s, st.copy(), cycles)["states"]
+    assert np.array_equal(got, ref)
+
+
 def test_reference_symbol_entry_point():
     nl, comp = design("c1", level_split=6, **CONFIG1)
     s = comp.flatten(16)
