@@ -293,15 +293,22 @@ __device__ void run_partition(const ScanArgs &A, const u32 *__restrict__ bd,
           if (is_x) x[k] = ld_coherent(base + xoff + w);
         }
       }
+      // pext, lowest mask bit first into the accumulator (kernel_v1_impl.cuh:
+      // 112-119): one ctz, one bit-field extract and one shift-or per bit;
+      // the X lane's work only in X-capable partitions
+      if (is_x) {
 #pragma unroll
-      for (u32 k = 0; k < kPreRounds; ++k) {
-        u32 m = e[k].y;
-        while (m) {
-          u32 low = m & (0u - m);
-          acc_v = (acc_v << 1) | ((v[k] & low) ? 1u : 0u);
-          acc_x = (acc_x << 1) | ((x[k] & low) ? 1u : 0u);
-          m ^= low;
-        }
+        for (u32 k = 0; k < kPreRounds; ++k)
+          for (u32 m = e[k].y; m; m &= m - 1) {
+            const u32 p = __builtin_ctz(m);
+            acc_v = (acc_v << 1) | ((v[k] >> p) & 1u);
+            acc_x = (acc_x << 1) | ((x[k] >> p) & 1u);
+          }
+      } else {
+#pragma unroll
+        for (u32 k = 0; k < kPreRounds; ++k)
+          for (u32 m = e[k].y; m; m &= m - 1)
+            acc_v = (acc_v << 1) | ((v[k] >> __builtin_ctz(m)) & 1u);
       }
     }
     for (u32 r = kPreRounds; r < rounds; ++r) {  // rare: very wide reads
@@ -310,12 +317,10 @@ __device__ void run_partition(const ScanArgs &A, const u32 *__restrict__ bd,
       const u32 *base = (er.x >> 31) ? out : in;
       u32 w = er.x & 0x7fffffffu;
       u32 v = ld_coherent(base + w), x = is_x ? ld_coherent(base + xoff + w) : 0u;
-      u32 m = er.y;
-      while (m) {
-        u32 low = m & (0u - m);
-        acc_v = (acc_v << 1) | ((v & low) ? 1u : 0u);
-        acc_x = (acc_x << 1) | ((x & low) ? 1u : 0u);
-        m ^= low;
+      for (u32 m = er.y; m; m &= m - 1) {
+        const u32 p = __builtin_ctz(m);
+        acc_v = (acc_v << 1) | ((v >> p) & 1u);
+        acc_x = (acc_x << 1) | ((x >> p) & 1u);
       }
     }
     pi = p_st;
