@@ -248,18 +248,24 @@ class BatchedRun:
         """Slot-cycles the groups ran (32 per group cycle)."""
         return int(self.group_cycles().sum()) * 32
 
-    def lane_outputs(self):
-        """{lane: uint8[cycles, n_obs]} observed values after every cycle."""
+    def lane_outputs(self, lanes=None, max_cycles=None):
+        """{lane: uint8[cycles, n_obs]} observed values after every cycle,
+        for the given lanes (all by default) and at most max_cycles cycles
+        each; only those rows of the device buffer are copied."""
         n_obs = len(self.obs_bits)
-        obs = self.d["obs"].cpu().numpy().view(np.uint32)
-        obs = obs[:self.G * self.obs_cap * n_obs].reshape(self.G, self.obs_cap, n_obs)
+        obs = self.d["obs"][:self.G * self.obs_cap * n_obs].view(self.G, self.obs_cap, n_obs)
         info = self.d["info"].cpu().numpy().view(np.uint32)
+        where = {lane: k for k, lane in enumerate(self.order.tolist())}
         out = {}
-        for k, lane in enumerate(self.order.tolist()):
+        for lane in (self.order.tolist() if lanes is None else lanes):
+            k = where[int(lane)]
             gs, start = int(info[2 * k]), int(info[2 * k + 1])
             g, l = divmod(gs, 32)
             c = int(self.cycles[lane])
-            out[lane] = ((obs[g, start:start + c, :] >> np.uint32(l)) & 1).astype(np.uint8)
+            if max_cycles is not None:
+                c = min(c, int(max_cycles))
+            rows = obs[g, start:start + c, :].cpu().numpy().view(np.uint32)
+            out[int(lane)] = ((rows >> np.uint32(l)) & 1).astype(np.uint8)
         return out
 
 
