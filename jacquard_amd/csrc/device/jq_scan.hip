@@ -479,12 +479,18 @@ __device__ void run_partition(const ScanArgs &A, const u32 *__restrict__ bd,
     {
       const uint4 f = ckf;
       if ((u32)t < num_ios) {
-        u32 ck = (gather_bits(wo_v, ck0, ck1, ck2, ck3) & ~f.y) ^ f.x;
+        // a word whose clock-enable bits are all forced (set0 all ones,
+        // every combinational writeout) needs no gather: ck = inv, ckx = 0;
+        // only words holding DFFs pay for the 32 LDS reads
+        const bool gated = f.y != 0xFFFFFFFFu;
+        u32 ck = f.x;
+        if (gated) ck = (gather_bits(wo_v, ck0, ck1, ck2, ck3) & ~f.y) ^ f.x;
         u32 data = wo_v[t] ^ f.z;
         u32 w = io_offset + (u32)t;
         st_coherent(out + w, (old_v & ~ck) | (data & ck));
         if (is_x) {
-          u32 ckx = gather_bits(wo_x, ck0, ck1, ck2, ck3) & ~f.y;
+          u32 ckx = 0;
+          if (gated) ckx = gather_bits(wo_x, ck0, ck1, ck2, ck3) & ~f.y;
           st_coherent(out + xoff + w, (old_x & ~ck & ~ckx) | (wo_x[t] & ck & ~ckx) | ckx);
         }
         if (TIMING) {
