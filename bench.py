@@ -75,6 +75,9 @@ def parse():
                     help="level-split period of the batched form's script (it runs "
                          "more partitions per block, so it prefers shorter major stages)")
     ap.add_argument("--blocks", type=int, default=0, help="plain blocks (0 = auto)")
+    ap.add_argument("--no-plain-tune", action="store_true",
+                    help="plain script with the compiler's default partition width "
+                         "instead of the measured choice (sim.tune_plain)")
     ap.add_argument("--lanes", type=int, default=256)
     ap.add_argument("--batched-scale", type=float, default=1.0,
                     help="scale of the batched per-lane cycle counts "
@@ -754,10 +757,18 @@ def main():
     nl, comp = build_design(a)
     res = ctypes.c_int(0)
     N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
-    # one block per partition of the widest major stage: more blocks only
-    # add grid-barrier arrivals, fewer serialise partitions
-    blocks = a.blocks or min(max(comp.parts_per_stage()), res.value)
-    script = comp.flatten(blocks)
+    tune_table = None
+    if a.blocks:
+        script = comp.flatten(a.blocks)
+    elif a.no_plain_tune:
+        # one block per partition of the widest major stage: more blocks only
+        # add grid-barrier arrivals, fewer serialise partitions
+        script = comp.flatten(min(max(comp.parts_per_stage()), res.value))
+    else:
+        from jacquard_amd import sim
+        t = time.time()
+        comp, script, tune_table = sim.tune_plain(nl, a.split, log=log)
+        log(f"plain tune: {time.time() - t:.1f}s")
     log(f"plain script: {script.num_blocks} blocks, {script.num_major_stages} major stages, "
         f"{script.num_parts} partitions, {script.blocks_data.nbytes / 1e6:.1f} MB")
     p = bench_plain(a, script, torch)
@@ -835,6 +846,7 @@ def main():
             "major_stages": script.num_major_stages,
             "partitions": script.num_parts,
             "reg_io_state_words": script.reg_io_state_size,
+            "plain_partition_tune": tune_table,
             "script_bytes": int(script.blocks_data.nbytes),
             "forms": forms,
         },

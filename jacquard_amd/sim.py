@@ -128,3 +128,64 @@ def canonical_events(ev):
         return np.zeros(0, EVENT_DTYPE)
     order = np.lexsort((ev["event_type"], ev["message_id"], ev["cycle"]))
     return ev[order]
+
+
+def tune_plain(netlist, level_split, caps=(2600, 1800, 1300, 1100, 900, 800, 700),
+               trial_cycles=300, seed=1, log=None):
+    """Partition width for the plain drop-in, chosen by measurement.
+
+    The plain kernel runs one workgroup per partition of the widest major
+    stage, and a major stage takes as long as its slowest partition, whose
+    time grows with its boomerang-stage count.  Capping the nodes per logic
+    level of a partition (``max_level_nodes``) gives more, narrower
+    partitions with fewer boomerang stages each, up to the point where the
+    widest major stage no longer fits the co-resident workgroups.  Every cap
+    whose widest stage fits is compiled and timed over ``trial_cycles``
+    cycles; the fastest wins.  Returns (Compiled, script, table)."""
+    from .script import Compiled
+    from . import stimulus
+    torch = _torch()
+    res = ctypes.c_int(0)
+    N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
+    stream = torch.cuda.current_stream()
+    best, table = None, []
+    for cap in caps:
+        comp = Compiled(netlist, level_split=level_split, max_level_nodes=cap)
+        widest = max(comp.parts_per_stage())
+        if widest > res.value:
+            table.append({"max_level_nodes": cap, "widest_stage": widest, "fits": False})
+            continue
+        s = comp.flatten(widest)
+        ss = s.effective_state_size()
+        st = to_device(stimulus.build_states(s, trial_cycles, seed=seed, p=0.1))
+        d_bs = to_device(np.ascontiguousarray(s.blocks_start, np.uint64))
+        d_bd = to_device(np.ascontiguousarray(s.blocks_data, np.uint32))
+        one = torch.zeros(max(s.sram_storage_size, 1), dtype=torch.int32, device="cuda")
+
+        def launch(n):
+            N.check(N.hip().jq_simulate_scan(
+                s.num_blocks, s.num_major_stages, d_bs.data_ptr(), d_bd.data_ptr(),
+                one.data_ptr(), one.data_ptr(), n, ss, st.data_ptr(), None, None, 0, 0,
+                ctypes.c_void_p(stream.cuda_stream)))
+
+        launch(min(20, trial_cycles))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        launch(trial_cycles)
+        e1.record(stream)
+        e1.synchronize()
+        if device_error():
+            raise RuntimeError("device barrier timeout while tuning the plain script")
+        rate = trial_cycles / (e0.elapsed_time(e1) / 1e3)
+        table.append({"max_level_nodes": cap, "widest_stage": widest, "fits": True,
+                      "partitions": s.num_parts, "cycles_per_s": rate})
+        if log:
+            log(f"plain tune: max_level_nodes {cap}: {s.num_parts} partitions, widest stage "
+                f"{widest}, {rate:,.0f} cycles/s")
+        if best is None or rate > best[2]:
+            best = (comp, s, rate)
+        del st, d_bs, d_bd, one
+    if best is None:
+        raise RuntimeError("no partition width fits the co-resident workgroups")
+    return best[0], best[1], table
