@@ -87,6 +87,9 @@ def parse():
     ap.add_argument("--batched-groups", type=int, default=0,
                     help="lane groups of 32 slots (0 = one per 32 lanes)")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-width-tune", action="store_true",
+                    help="batched form with the compiler's default partition width "
+                         "instead of the measured choice (batched.tune_partition_width)")
     ap.add_argument("--no-learned", action="store_true",
                     help="batched form with the heuristic block assignment only")
     ap.add_argument("--no-cpu", action="store_true")
@@ -196,7 +199,7 @@ def bench_plain(a, script, torch):
     return out
 
 
-def bench_batched(a, comp, torch, cycles=None):
+def bench_batched(a, comp, torch, cycles=None, split=None):
     """Batched form on the same netlist: --lanes lanes with per-lane seeds,
     cycle counts U[10K, 200K] x --batched-scale and toggle rates
     U[0.02, 0.5], in one persistent launch with the device lane queue,
@@ -221,6 +224,12 @@ def bench_batched(a, comp, torch, cycles=None):
         tune = B.autotune(comp, cycles, log=log)
         W, G = tune["blocks"], tune["groups"]
         log(f"autotune: chose {G} groups x {W} blocks in {time.time() - t:.1f}s")
+    width = None
+    if not a.no_width_tune:
+        t = time.time()
+        comp, width = B.tune_partition_width(
+            comp, a.batched_split if split is None else split, W, G, log=log)
+        log(f"width tune: {time.time() - t:.1f}s")
     learned = None
     costs = None
     if not a.no_learned:
@@ -273,7 +282,7 @@ def bench_batched(a, comp, torch, cycles=None):
     return dict(wall=wall, kernel_ms=e0.elapsed_time(e1), lane_cycles=lc,
                 slot_cycles=run.slot_cycles_total(), groups=G, blocks=W,
                 verified_cycles=checked, script=script, learned=learned,
-                tune=None if tune is None else tune["trials"])
+                tune=None if tune is None else tune["trials"], width=width)
 
 
 def bench_xtiming(a, torch):
@@ -500,7 +509,7 @@ def bench_mult16m(a, torch):
                                       B.max_resident_batched() // groups))
     a.batched_groups = groups
     cycles = np.full(a.lanes, a.cycles)
-    b = bench_batched(a, comp, torch, cycles=cycles)
+    b = bench_batched(a, comp, torch, cycles=cycles, split=a.split)
     value = b["lane_cycles"] / b["wall"]
     res = ctypes.c_int(0)
     N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
@@ -800,6 +809,7 @@ def main():
                             "slot_utilisation": b["lane_cycles"] / b["slot_cycles"],
                             "kernel_s": b["kernel_ms"] / 1e3,
                             "autotune": b["tune"],
+                            "partition_width": b["width"],
                             "learned_cost_model": b["learned"],
                             "launch": "one hipGraph replay (reset + persistent kernel)",
                             "verified_cycles": b["verified_cycles"]}

@@ -345,3 +345,39 @@ def autotune(compiled, lane_cycles, widths=(16, 24, 32, 48, 64), trial_cycles=24
         raise RuntimeError("autotune: no candidate fits the device")
     return dict(groups=best[2], blocks=best[1], script=best[3], us_per_cycle=best[4],
                 trials=trials)
+
+
+def tune_partition_width(compiled, level_split, blocks, groups,
+                         caps=(1800, 1300, 1100, 900), trial_cycles=48, log=None):
+    """Partition width for the batched form at a chosen (blocks, groups).
+
+    Capping the nodes per logic level of a partition (``max_level_nodes``)
+    gives more, narrower partitions with fewer boomerang stages each; once a
+    major stage holds more partitions than the group has blocks they
+    serialise.  The given compilation and every cap whose widest major stage
+    is at most 1.5x the block count are flattened for `blocks` and timed over
+    `trial_cycles` cycles of `groups` full groups; the fastest wins.
+    Returns (Compiled, table)."""
+    from .costmodel import batched_us_per_cycle
+    from .script import Compiled
+    table = []
+    us = batched_us_per_cycle(compiled.flatten_shared(blocks), groups, cycles=trial_cycles)
+    table.append({"max_level_nodes": None, "widest_stage": max(compiled.parts_per_stage()),
+                  "tried": True, "us_per_cycle": us})
+    best = (compiled, us)
+    if log:
+        log(f"width tune: as compiled: {us:.1f} us/cycle")
+    for cap in caps:
+        comp = Compiled(compiled.netlist, level_split=level_split, max_level_nodes=cap)
+        widest = max(comp.parts_per_stage())
+        row = {"max_level_nodes": cap, "widest_stage": widest}
+        if widest > 1.5 * blocks:
+            table.append(dict(row, tried=False))
+            continue
+        us = batched_us_per_cycle(comp.flatten_shared(blocks), groups, cycles=trial_cycles)
+        table.append(dict(row, tried=True, us_per_cycle=us))
+        if log:
+            log(f"width tune: max_level_nodes {cap}: widest stage {widest}, {us:.1f} us/cycle")
+        if us < best[1]:
+            best = (comp, us)
+    return best[0], table
