@@ -348,7 +348,7 @@ def autotune(compiled, lane_cycles, widths=(16, 24, 32, 48, 64), trial_cycles=24
 
 
 def tune_partition_width(compiled, level_split, blocks, groups,
-                         caps=(1800, 1300, 1100, 900), trial_cycles=48, log=None):
+                         caps=(4000, 1800, 1300, 1100, 900), trial_cycles=48, log=None):
     """Partition width for the batched form at a chosen (blocks, groups).
 
     Capping the nodes per logic level of a partition (``max_level_nodes``)
