@@ -779,7 +779,8 @@ def main():
         comp, script, tune_table = sim.tune_plain(nl, a.split, log=log)
         log(f"plain tune: {time.time() - t:.1f}s")
     log(f"plain script: {script.num_blocks} blocks, {script.num_major_stages} major stages, "
-        f"{script.num_parts} partitions, {script.blocks_data.nbytes / 1e6:.1f} MB")
+        f"{script.num_parts} partitions, {script.blocks_data.nbytes / 1e6:.1f} MB "
+        f"({script.blocks_data.nbytes} bytes)")
     p = bench_plain(a, script, torch)
     value = p["cycles"] / p["wall"]
     avg_launch_s = float(np.mean(p["launch_ms"])) / 1e3
