@@ -142,21 +142,6 @@ def stage_costs(compiled, script, per_part):
     return costs
 
 
-def heuristic_costs(compiled):
-    return [np.asarray(s, np.float64) + 3.0 for s in compiled.stage_counts()]
-
-
-def predicted_makespan(script, per_part):
-    """Sum over major stages of the most loaded block (unit of per_part)."""
-    tot = 0.0
-    for m in range(script.num_major_stages):
-        loads = np.zeros(script.num_blocks)
-        sel = script.parts[:, 0] == m
-        np.add.at(loads, script.parts[sel, 1].astype(np.int64), per_part[sel])
-        tot += loads.max() if loads.size else 0.0
-    return tot
-
-
 # ----------------------------------------------------------------------
 # batched form: samples, training set, assignment A/B
 
@@ -198,21 +183,6 @@ def training_set(compiled, blocks_list, groups, sample_cycles, skip=2, seed=0):
             med.setdefault((mj, idx), []).append(v)
     med = {k: float(np.median(v)) for k, v in med.items()}
     return np.concatenate(xs).astype(np.float32), np.concatenate(ys), med
-
-
-def predicted_costs(compiled, model, blocks):
-    """Per-partition costs [major][index] predicted by `model` from the
-    features of a flattening (any block count)."""
-    s = compiled.flatten(blocks)
-    y = model.predict(s.part_features())
-    return stage_costs(compiled, s, y), s
-
-
-def measured_costs(compiled, med):
-    costs = [np.zeros(n) for n in compiled.parts_per_stage()]
-    for (mj, idx), v in med.items():
-        costs[mj][idx] = v
-    return costs
 
 
 def batched_us_per_cycle(script, groups, cycles=48, seed=1):

@@ -6,11 +6,15 @@
 // to CDNA4 as follows:
 //   - script sections are streamed with 16-byte per-thread loads (each
 //     section is one fully coalesced 4 KiB request), the next boomerang
-//     stage's permutation words are in flight while the current tree runs;
+//     stage's permutation words are in flight while the current tree runs,
+//     and a partition's prologue loads are issued before the barrier that
+//     precedes it;
+//   - snapshot words move between workgroups only through agent-coherent
+//     loads and stores, so the grid barrier carries no cache maintenance;
 //   - the 8192-bit shuffle is 32 LDS bit-gathers per thread on all 4 waves;
 //   - the whole AND tree (hier 0..7 and the in-word tail) runs in wave 0
-//     with cross-lane moves, so a boomerang stage costs two workgroup
-//     barriers instead of one per tree level;
+//     with permlane / DPP cross-lane moves, so a boomerang stage costs two
+//     workgroup barriers instead of one per tree level;
 //   - X masks and u16 arrivals ride in the same wave-0 pass when enabled.
 // The computed values are bit-identical to the reference CPU interpreter
 // (src/sim/cpu_reference.rs:17-674) and to the CUDA kernel's X, arrival and

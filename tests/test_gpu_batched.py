@@ -106,3 +106,18 @@ def test_batched_wide_group_hierarchical_barrier():
     run = BatchedRun(s, seeds, probs, cycles, groups=1)
     print("blocks", s.num_blocks, "major", s.num_major_stages, flush=True)
     _check(s, run, seeds, probs, cycles, range(40))
+
+
+def test_width_tuner_keeps_lanes_exact():
+    """batched.tune_partition_width picks among cuts of the same netlist;
+    the kept one runs the lanes bit-exactly."""
+    from jacquard_amd.batched import tune_partition_width
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    comp2, table = tune_partition_width(comp, 6, 16, 1, caps=(60, 40), trial_cycles=16)
+    print("width table", table, flush=True)
+    assert table[0]["max_level_nodes"] is None and table[0]["tried"]
+    assert [r["tried"] for r in table[1:]] == [True, False]  # widest 13 and 25 vs 24
+    s = comp2.flatten(16)
+    seeds, probs, cycles = _lanes(32, 10, 40, 9)
+    run = BatchedRun(s, seeds, probs, cycles, groups=1)
+    _check(s, run, seeds, probs, cycles, range(32))

@@ -173,3 +173,40 @@ def test_device_output_changes_match_host_writer(xprop):
     dev = vcd.write_output_vcd_changes(s, offs, cnt, rec)
     assert dev == host
     assert len(rec) < S[1:].size  # compacted: far fewer words than snapshots
+
+
+def test_tune_plain_chooses_a_fitting_width_that_stays_exact():
+    """The measured partition-width choice only changes how the netlist is
+    cut: whatever cap it keeps, the script fits the co-resident workgroups
+    and simulates bit-exactly."""
+    from jacquard_amd.netlist import Netlist
+    nl = Netlist.random(seed=5, gates=20000, depth=40, dffs=1024, inputs=64, outputs=32,
+                        geometric_p=0.3, invert_prob=0.5)
+    comp, s, table = sim.tune_plain(nl, 4, caps=(2600, 400, 200), trial_cycles=20)
+    _say("tune table", table)
+    assert any(r["fits"] for r in table)
+    assert s.num_blocks == max(comp.parts_per_stage())
+    cycles = 60
+    st = stimulus.build_states(s, cycles, seed=31, p=0.3)
+    ref = st.copy()
+    oracle.run(s, ref, cycles)
+    got = sim.run_scan(s, st.copy(), cycles)["states"]
+    assert np.array_equal(got, ref)
+
+
+def test_scan_profiled_samples_every_partition_and_keeps_values():
+    """jq_simulate_scan_profiled writes a positive tick count for every
+    partition execution of the sampled cycles and simulates exactly like the
+    unprofiled kernel."""
+    from jacquard_amd import costmodel as CM
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    s = comp.flatten(8)
+    cycles, sampled = 40, 10
+    st = stimulus.build_states(s, cycles, seed=41, p=0.4)
+    ref = st.copy()
+    oracle.run(s, ref, cycles)
+    d = sim.to_device(st.copy())
+    ticks = CM.profile_partitions(s, d, cycles, sampled)
+    assert np.array_equal(d.cpu().numpy().view(np.uint32), ref)
+    assert ticks.shape == (sampled, s.num_parts)
+    assert (ticks > 0).all()
