@@ -210,3 +210,25 @@ def test_scan_profiled_samples_every_partition_and_keeps_values():
     assert np.array_equal(d.cpu().numpy().view(np.uint32), ref)
     assert ticks.shape == (sampled, s.num_parts)
     assert (ticks > 0).all()
+
+
+@pytest.mark.parametrize("xprop", [False, True])
+def test_global_read_wider_than_prefetched_rounds(xprop):
+    """Scripts whose global read needs more rounds than the kernel prefetches
+    (up to 32 single-bit rounds per thread here): the extra rounds are read
+    after the prefetched ones, with the X lane when X-capable."""
+    from helpers import OneBitRounds
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    nl = Netlist.random(seed=5, gates=20000, depth=40, dffs=1024, inputs=64, outputs=32,
+                        geometric_p=0.3, invert_prob=0.5)
+    s = Compiled(nl, level_split=4, max_part_nodes=500).flatten(8, xprop=xprop)
+    w = OneBitRounds(s)
+    _say("xprop", xprop, "max rounds", w.max_rounds)
+    assert w.max_rounds > 16
+    cycles = 60
+    st = stimulus.build_states(s, cycles, seed=33, p=0.3)
+    ref = st.copy()
+    oracle.run(s, ref, cycles)  # the original script: same values
+    got = sim.run_scan(w, st.copy(), cycles)["states"]
+    assert np.array_equal(got, ref)
