@@ -122,6 +122,39 @@ def test_timing_events():
     assert np.array_equal(a, b)
 
 
+def test_timed_reference_symbol_keeps_1024_events():
+    """simulate_v1_noninteractive_timed_hip keeps the reference buffer's
+    1024 records (csrc/event_buffer.h:99-111): the header counts every
+    violation, sets the overflow flag, and the kept records are violations
+    of the full list."""
+    nl, comp = design("tm", level_split=6, seed=6, gates=8000, depth=30,
+                      dffs=800, inputs=32, outputs=16, geometric_p=0.3)
+    d = np.zeros(nl.counts()["num_pins"] + 1, np.uint16)
+    rng = np.random.default_rng(1)
+    A = nl.arrays()
+    ands = A["drv"] == 1
+    d[ands] = rng.integers(20, 201, ands.sum())
+    nl.set_delays(d)
+    s = comp.flatten(16)
+    s.inject_timing()
+    s.enable_timing_arrivals()
+    cons = s.build_timing_constraint_buffer(190, 50, 20)
+    cycles = 60
+    st = stimulus.build_states(s, cycles, seed=15, p=0.5)
+    ref = st.copy()
+    ev_ref, n_ref = oracle.run(s, ref, cycles, constraints=cons,
+                               arrival_state_offset=s.arrival_state_offset,
+                               event_capacity=1 << 20)
+    assert n_ref > 1024
+    out = sim.run_scan(s, st.copy(), cycles, constraints=cons, reference_symbols=True)
+    assert np.array_equal(out["states"], ref)
+    assert out["event_count"] == n_ref and out["overflow"] == 1
+    kept = sim.canonical_events(out["events"])
+    assert len(kept) == 1024
+    full = {bytes(e) for e in sim.canonical_events(ev_ref)}
+    assert all(bytes(e) in full for e in kept)
+
+
 def test_xprop_timing_srams_together():
     """X lane, arrivals, setup/hold and SRAMs in one run (the 4M-gate
     workload's feature mix at test size)."""
