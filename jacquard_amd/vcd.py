@@ -327,6 +327,29 @@ def write_output_vcd(script, offsets_timestamps, states, timescale=(1, "ps"),
     return "\n".join(lines) + "\n"
 
 
+def write_lane_vcd(script, obs, obs_lits, timestamps, timescale=(1, "ps"),
+                   scope="gem_top_module"):
+    """Output VCD text of one batched lane: obs is the lane's
+    uint8[cycles, len(obs_lits)] (BatchedRun.lane_outputs), row c the
+    outputs after cycle c, emitted at timestamps[c].  The text is the one
+    write_output_vcd writes for the same lane run through the plain path
+    with snapshot c + 1 emitted at timestamps[c]; every output port that is
+    not a constant or an input pass-through must be in obs_lits."""
+    mapping = output_mapping(script)
+    col = {lit: k for k, lit in enumerate(obs_lits)}
+    lines = _header_text(timescale, scope, mapping)
+    last = [2] * len(mapping)
+    for c, ts in enumerate(timestamps):
+        lines.append(f"#{ts}")
+        for i, (lit, pos, _) in enumerate(mapping):
+            v = lit if pos is None else int(obs[c, col[lit]])
+            if v == last[i]:
+                continue
+            last[i] = v
+            lines.append(f"{_SYM[v]}{id_code(i)}")
+    return "\n".join(lines) + "\n"
+
+
 CONST0_BIT = 0xFFFFFFFF
 CONST1_BIT = 0xFFFFFFFE
 

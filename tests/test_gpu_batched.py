@@ -121,3 +121,24 @@ def test_width_tuner_keeps_lanes_exact():
     seeds, probs, cycles = _lanes(32, 10, 40, 9)
     run = BatchedRun(s, seeds, probs, cycles, groups=1)
     _check(s, run, seeds, probs, cycles, range(32))
+
+
+def test_lane_vcd_matches_plain_path_vcd():
+    """A batched lane's output VCD is the text the plain path writes for the
+    same stimulus (whose writer is pinned byte for byte by the reference's
+    golden VCDs)."""
+    from jacquard_amd import vcd
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    s = comp.flatten(8)
+    seeds, probs, cycles = _lanes(40, 20, 80, 11)
+    run = BatchedRun(s, seeds, probs, cycles)
+    outs = run.run()
+    rio = s.reg_io_state_size
+    for lane in (0, 17, 39):
+        c = int(cycles[lane])
+        ts = [10 * (k + 1) for k in range(c)]
+        got = vcd.write_lane_vcd(s, outs[lane], run.obs_lits, ts)
+        st = stimulus.build_states(s, c, seed=int(seeds[lane]), p=float(probs[lane]))
+        oracle.run(s, st, c)
+        ref = vcd.write_output_vcd(s, [((k + 1) * rio, ts[k]) for k in range(c)], st)
+        assert got == ref, f"lane {lane}"
