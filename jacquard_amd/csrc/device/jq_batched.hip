@@ -358,7 +358,9 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
       const u32 *i0 = in + (size_t)io * 32;
       for (u32 q = t; q < num_ios * 32; q += 256) {
         uint4 f = *reinterpret_cast<const uint4 *>(fl + 4 * (q >> 5));
-        u32 ck = (WO[sw(perm_at(sec, q))] & ~fbit(f.y, q)) ^ fbit(f.x, q);
+        // a bit whose enable is forced (set0) needs no gather: ck = inv
+        u32 ck = fbit(f.x, q);
+        if (!((f.y >> (q & 31)) & 1u)) ck ^= WO[sw(perm_at(sec, q))];
         u32 d = WO[sw(q)] ^ fbit(f.z, q);
         st_coherent(o + q, (ld_coherent(i0 + q) & ~ck) | (d & ck));
       }
