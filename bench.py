@@ -194,6 +194,16 @@ def bench_plain(a, script, torch):
         out["verified_cycles"] = v if np.array_equal(got, ref) else -1
         if out["verified_cycles"] < 0:
             raise AssertionError("plain GPU run differs from the CPU oracle")
+        # and the last cycles of the run: the oracle continues from the GPU's
+        # snapshot `total - v` (every word of it, including DFF state) with
+        # the same inputs
+        tail = st[total - v:total + 1].cpu().numpy().view(np.uint32).reshape(-1)
+        ref = tail.copy()
+        ref.reshape(v + 1, ss)[1:, nw:] = 0
+        oracle.run(script, ref, v)
+        if not np.array_equal(tail, ref):
+            raise AssertionError("plain GPU run differs from the CPU oracle at its end")
+        out["verified_cycles"] += v
     del st
     torch.cuda.empty_cache()
     return out
