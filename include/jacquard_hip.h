@@ -166,6 +166,16 @@ int jq_simulate_batched_profiled(
  * scan kernel (kind 0) or the batched kernel (kind 1) on the current device. */
 int jq_max_resident_blocks(int kind, int *out_blocks);
 
+/* Device barrier time-out flag (no counterpart in the reference): *out is
+ * nonzero when a grid barrier wait of a scan or batched launch gave up after
+ * its bounded spin since the flag was last cleared.  Launches return before
+ * their kernels finish, so callers read the flag after synchronising; the
+ * results of a launch that timed out are not valid.  With clear != 0 the flag is
+ * reset, and when it was set every barrier slot of the kind that timed out
+ * is zeroed as well, so later launches start from a consistent barrier
+ * state; clear only while no launch runs on the device. */
+int jq_device_error_flag(u32 *out, int clear);
+
 /* ---- Output change extraction (no counterpart in the reference) ---------
  * Replaces the host-side scan of every snapshot in write_output_vcd_impl
  * (src/sim/vcd_io.rs:758-820): for each of num_snapshots emitted snapshots

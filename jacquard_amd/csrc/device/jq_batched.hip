@@ -497,6 +497,15 @@ int batched_error_flag(u32 *out, int clear) {
   if (clear) {
     u32 z = 0;
     if (hipMemcpyToSymbol(HIP_SYMBOL(gb_error), &z, 4) != hipSuccess) return -1;
+    // a timed-out wait leaves partial arrival counts behind in its slot:
+    // start every slot over (no batched launch may be running)
+    if (v) {
+      void *bar = nullptr;
+      if (hipGetSymbolAddress(&bar, HIP_SYMBOL(gb_barrier)) != hipSuccess ||
+          hipMemset(bar, 0, sizeof(u32) * kBatchedSlots * kCohSlotWords) != hipSuccess ||
+          hipDeviceSynchronize() != hipSuccess)
+        return -1;
+    }
   }
   *out = v;
   return 0;
@@ -588,8 +597,14 @@ int batched_impl(usize num_groups, usize blocks_per_group, usize num_major_stage
   if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(gb_barrier)) != hipSuccess) {
     return set_error("hipGetSymbolAddress(gb_barrier) failed");
   }
-  u32 first = __atomic_fetch_add(&b_slot, (u32)num_groups, __ATOMIC_RELAXED) % kBatchedSlots;
-  if (first + num_groups > (usize)kBatchedSlots) first = 0;
+  // the launch's groups take consecutive slots from the rotation; a run that
+  // would pass the end starts at slot 0, and the rotation continues after it
+  u32 cur = __atomic_load_n(&b_slot, __ATOMIC_RELAXED), first = 0;
+  do {
+    first = cur % kBatchedSlots;
+    if (first + num_groups > (usize)kBatchedSlots) first = 0;
+  } while (!__atomic_compare_exchange_n(&b_slot, &cur, first + (u32)num_groups, true,
+                                        __ATOMIC_RELAXED, __ATOMIC_RELAXED));
   A.bar = bar_base + kCohSlotWords * first;
   void *args[] = {&A};
   // Under stream capture (a hipGraph of the run) the launch is an ordinary

@@ -744,6 +744,15 @@ extern "C" int jq_device_error_flag(u32 *out, int clear) {
   if (clear) {
     u32 z = 0;
     if (hipMemcpyToSymbol(HIP_SYMBOL(g_error), &z, 4) != hipSuccess) return -1;
+    // a timed-out wait leaves partial arrival counts behind in its slot:
+    // start every slot over (no scan launch may be running)
+    if (v) {
+      void *bar = nullptr;
+      if (hipGetSymbolAddress(&bar, HIP_SYMBOL(g_barrier)) != hipSuccess ||
+          hipMemset(bar, 0, sizeof(u32) * kCohSlots * kCohSlotWords) != hipSuccess ||
+          hipDeviceSynchronize() != hipSuccess)
+        return -1;
+    }
   }
   if (batched_error_flag(&vb, clear)) return -1;
   *out = v | vb;

@@ -104,12 +104,13 @@ void *jqh_vcd_parse_body(const char *body, uint64_t len, const char *codes,
     if (c0 == '#') {
       uint64_t t = 0;
       for (size_t k = 1; k < tok.size(); ++k) {
-        if (tok[k] < '0' || tok[k] > '9') {
+        const uint64_t d = (uint64_t)(tok[k] - '0');
+        if (tok[k] < '0' || tok[k] > '9' || t > (kU64Max - d) / 10) {
           t_err = "vcd: bad timestamp " + std::string(tok);
           delete out;
           return nullptr;
         }
-        t = t * 10 + (uint64_t)(tok[k] - '0');
+        t = t * 10 + d;
       }
       if (t == vcd_time) continue;
       if (last_active) {

@@ -45,7 +45,7 @@ def from_device(t, dtype):
 
 def device_error(clear=True):
     v = ctypes.c_uint32(0)
-    N.hip().jq_device_error_flag(ctypes.byref(v), int(clear))
+    N.check(N.hip().jq_device_error_flag(ctypes.byref(v), int(clear)))
     return v.value
 
 

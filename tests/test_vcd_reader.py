@@ -103,3 +103,9 @@ def test_empty_and_header_only_sections():
 def test_bad_timestamp_raises():
     with pytest.raises(ValueError):
         vcd.parse_body("#12x 1!", {"!": {0: (0, None)}}, [], 1)
+    # a time that does not fit 64 bits is an error, not a wrapped value
+    with pytest.raises(ValueError):
+        vcd.parse_body("#18446744073709551616 1!", {"!": {0: (0, None)}}, [], 1)
+    states, offs = vcd.parse_body("#0 1! #18446744073709551615 0!",
+                                  {"!": {0: (0, None)}}, [], 1)[:2]
+    assert states.tolist() == [1, 0] and len(offs) == 1
