@@ -202,6 +202,36 @@ int jq_output_changes(const u32 *states, const usize *snapshot_offsets,
  * the caller times it. */
 int jq_lds_read_probe(u32 num_blocks, u32 iters, u32 *sink, void *stream);
 
+/* ---- Reactive co-simulation edges ---------------------------------------
+ * Replaces the Metal co-simulation's per-edge GPU pipeline
+ * (src/sim/cosim_metal.rs:547-651 encoding csrc/kernel_v1.metal:678-1025):
+ * for num_edges edges, schedule edge k = (sched_offset + e) % num_sched:
+ *   1. the output snapshot states[ss .. 2ss) is copied to the input snapshot
+ *      states[0 .. ss) and the edge's bit operations are applied in order
+ *      (edge_ops: u32 pairs (state bit, value), edge k's pairs at
+ *      [edge_op_start[k], edge_op_start[k+1]); the reference's BitOp
+ *      layout, csrc/kernel_v1.metal:664-667);
+ *   2. the SPI flash's data outputs d_i are written to its data-in bits;
+ *   3. one cycle of the scan kernel over the two snapshots;
+ *   4. the SPI flash model steps on the edge's outputs;
+ *   5. with trace != NULL, both snapshots are copied to trace[e*2ss ..].
+ * state_size = ss (2-state, no timing).  flash_state: the reference's
+ * 48-byte FlashState (jq_cosim_flash_state_bytes(), csrc/kernel_v1.metal:
+ * 731-749) or NULL for no flash; flash_pins: 11 u32 = data-in bits d0..d3,
+ * SPI clock output bit, chip-select output bit, data-out bits d0..d3
+ * (0xFFFFFFFF: not connected), has_flash; flash_data: the flash image of
+ * flash_data_size bytes.  All pointers are device pointers; the host keeps
+ * FlashState.in_reset and the reset bit operations current between calls.
+ * Bit positions past the snapshot are ignored.  Asynchronous on `stream`. */
+size_t jq_cosim_flash_state_bytes(void);
+int jq_cosim_edges(usize num_blocks, usize num_major_stages,
+                   const usize *blocks_start, const u32 *blocks_data,
+                   u32 *sram_data, usize state_size, u32 *states,
+                   const u32 *edge_ops, const u32 *edge_op_start, u32 num_sched,
+                   u32 sched_offset, u32 num_edges, void *flash_state,
+                   const u32 *flash_pins, const u8 *flash_data,
+                   u32 flash_data_size, u32 *trace, void *stream);
+
 /* ---- Learned cost model MLP (no counterpart in the reference) -----------
  * Fully connected FP32 network in -> h1 -> h2 -> 1 with ReLU (in <= 32,
  * h1, h2 <= 64), row-major weights W1[h1][in], b1[h1], W2[h2][h1], b2[h2],

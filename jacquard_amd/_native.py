@@ -115,6 +115,9 @@ HIP_SYMBOLS = [
     "jq_max_resident_blocks",
     "jq_output_changes",
     "jq_lds_read_probe",
+    "jq_cosim_flash_state_bytes",
+    "jq_cosim_edges",
+    "jq_device_error_flag",
     "jq_mlp_forward",
     "jq_mlp_train_step",
     "jq_mlp_num_params",
@@ -155,6 +158,10 @@ def hip():
         "jq_output_changes": (ctypes.c_int, [c_vp, c_vp, c_u32, c_vp, c_u32,
                                              c_u32, c_vp, c_vp, c_vp, c_vp]),
         "jq_lds_read_probe": (ctypes.c_int, [c_u32, c_u32, c_vp, c_vp]),
+        "jq_cosim_flash_state_bytes": (ctypes.c_size_t, []),
+        "jq_cosim_edges": (ctypes.c_int, [
+            c_u64, c_u64, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_u32,
+            c_u32, c_u32, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp]),
         "jq_max_resident_blocks": (ctypes.c_int, [ctypes.c_int,
                                                   P(ctypes.c_int)]),
         "jq_mlp_forward": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int,

@@ -1,0 +1,249 @@
+"""Reactive co-simulation on the device: clock edges with host-free state
+preparation and an SPI flash model between them (the GPU edge pipeline of
+the reference's Metal co-simulation, src/sim/cosim_metal.rs, on the HIP scan
+kernel through jq_cosim_edges).
+
+  * `MultiClockScheduler` restates src/sim/cosim_metal.rs:1433-1508: the
+    edge schedule of several clock domains over one least common period at
+    the greatest common half-period resolution;
+  * `MultiClockScheduler.edge_ops` restates :1510-1611: the bit operations
+    of one schedule edge (reset, clock inputs, edge flags, constant inputs,
+    placeholders for model-driven inputs);
+  * `Cosim` owns the device buffers (two snapshots, the edge operations,
+    the flash state and image) and runs edges in batches that do not cross
+    the end of reset, updating the reset operation and the flash's reset
+    flag between batches (:3470-3497).
+"""
+import ctypes
+from dataclasses import dataclass, field
+from math import gcd
+
+import numpy as np
+
+from . import _native as N
+
+FLASH_STATE_BYTES = 48
+_IN_RESET_OFFSET = 36  # FlashState.in_reset (csrc/kernel_v1.metal:731-749)
+NC = 0xFFFFFFFF
+
+
+def _lcm(a, b):
+    return a // gcd(a, b) * b
+
+
+@dataclass
+class ClockTiming:
+    half_period_ps: int
+    phase_offset_ps: int = 0
+    domain_index: int = 0
+
+
+@dataclass
+class ClockDomain:
+    """State bits of one clock domain: its clock input (None: not an input)
+    and its posedge / negedge flag bits."""
+    clock_input_pos: int = None
+    posedge_flag_bits: list = field(default_factory=list)
+    negedge_flag_bits: list = field(default_factory=list)
+
+
+class MultiClockScheduler:
+    """Schedule of (falling, rising) per domain for every tick of one common
+    period (src/sim/cosim_metal.rs:1433-1508)."""
+
+    MAX_LEN = 1_000_000
+
+    def __init__(self, timings):
+        if not timings:
+            raise ValueError("at least one clock domain is required")
+        g = timings[0].half_period_ps
+        for t in timings[1:]:
+            g = gcd(g, t.half_period_ps)
+        for t in timings:
+            if t.phase_offset_ps > 0:
+                g = gcd(g, t.phase_offset_ps)
+        period = timings[0].half_period_ps * 2
+        for t in timings[1:]:
+            period = _lcm(period, t.half_period_ps * 2)
+        n = period // g
+        if n > self.MAX_LEN:
+            raise ValueError(f"multi-clock schedule too large ({n} ticks)")
+        ndom = max(t.domain_index + 1 for t in timings)
+        self.gcd_ps = g
+        self.schedule = []
+        for tick in range(n):
+            tps = tick * g
+            edges = [[False, False] for _ in range(ndom)]
+            for t in timings:
+                if tps < t.phase_offset_ps:
+                    continue
+                adj = tps - t.phase_offset_ps
+                if adj % t.half_period_ps:
+                    continue
+                if (adj // t.half_period_ps) % 2 == 0:
+                    edges[t.domain_index][0] = True   # falling
+                else:
+                    edges[t.domain_index][1] = True   # rising
+            self.schedule.append([tuple(e) for e in edges])
+
+    def __len__(self):
+        return len(self.schedule)
+
+    def edge_ops(self, tick, domains, reset_pos=None, reset_val=0, constants=(),
+                 model_driven=()):
+        """[(state bit, value)] of schedule tick `tick`, in the reference's
+        order: reset, per domain the clock input (falling: 0, rising: 1) and
+        its flags (posedge = rising, negedge = falling), constant inputs,
+        model-driven placeholders at 0."""
+        edges = self.schedule[tick % len(self.schedule)]
+        ops = []
+        if reset_pos is not None:
+            ops.append((reset_pos, int(reset_val)))
+        for i, d in enumerate(domains):
+            fall, rise = edges[i] if i < len(edges) else (False, False)
+            if d.clock_input_pos is not None:
+                if fall:
+                    ops.append((d.clock_input_pos, 0))
+                if rise:
+                    ops.append((d.clock_input_pos, 1))
+            ops += [(p, int(rise)) for p in d.posedge_flag_bits]
+            ops += [(p, int(fall)) for p in d.negedge_flag_bits]
+        ops += [(p, int(v)) for p, v in constants]
+        ops += [(p, 0) for p in model_driven]
+        return ops
+
+
+def clock_domains_of(script):
+    """One domain per clock input port of the netlist, with its posedge flag
+    (the netlist's DFFs are positive-edge)."""
+    ins, _ = script.netlist.ports()
+    out = []
+    for name, pin, flag in ins:
+        if flag:
+            out.append((name, ClockDomain(script.input_map[pin],
+                                          [script.input_map[flag]], [])))
+    return out
+
+
+@dataclass
+class FlashPins:
+    """State bits of the flash's signals (NC: not connected)."""
+    clk_out: int
+    csn_out: int
+    d_out: tuple = (NC, NC, NC, NC)
+    d_in: tuple = (NC, NC, NC, NC)
+
+    def words(self):
+        return np.asarray(list(self.d_in) + [self.clk_out, self.csn_out]
+                          + list(self.d_out) + [1], np.uint32)
+
+
+class Cosim:
+    """Edges of one script on the device: `run(num_edges)` advances the
+    simulation; `trace` collects both snapshots after every edge."""
+
+    def __init__(self, script, scheduler, domains, *, reset_pos=None, reset_active=1,
+                 reset_edges=0, constants=(), model_driven=(), flash_pins=None,
+                 flash_image=b"", stream=None):
+        import torch
+        self.torch = torch
+        self.script = script
+        if script.xprop_enabled or script.timing_arrivals_enabled:
+            raise ValueError("co-simulation runs the 2-state script")
+        self.ss = int(script.effective_state_size())
+        self.sched = scheduler
+        self.domains = list(domains)
+        self.reset_pos = reset_pos
+        self.reset_active = int(reset_active)
+        self.reset_edges = int(reset_edges)
+        self.constants = list(constants)
+        self.model_driven = list(model_driven)
+        self.edge = 0
+        self.stream = stream or torch.cuda.current_stream()
+        dev = "cuda"
+        self.d_bs = torch.from_numpy(np.ascontiguousarray(script.blocks_start, np.uint64)
+                                     .view(np.int64)).to(dev)
+        self.d_bd = torch.from_numpy(np.ascontiguousarray(script.blocks_data, np.uint32)
+                                     .view(np.int32)).to(dev)
+        self.d_sram = torch.zeros(max(script.sram_storage_size, 1), dtype=torch.int32,
+                                  device=dev)
+        self.states = torch.zeros(2 * self.ss, dtype=torch.int32, device=dev)
+        self.flash = flash_pins is not None
+        if self.flash:
+            if int(N.hip().jq_cosim_flash_state_bytes()) != FLASH_STATE_BYTES:
+                raise RuntimeError("FlashState size mismatch")
+            fs = np.zeros(FLASH_STATE_BYTES // 4, np.uint32)
+            b = fs.view(np.uint8)
+            b[8:12] = np.frombuffer(np.uint32(1).tobytes(), np.uint8)   # data_width
+            b[24:28] = np.frombuffer(np.uint32(1).tobytes(), np.uint8)  # prev_csn
+            b[28] = 0x0F                                                 # d_i
+            b[36:40] = np.frombuffer(np.uint32(1).tobytes(), np.uint8)  # in_reset
+            b[44:48] = np.frombuffer(np.uint32(1).tobytes(), np.uint8)  # model_prev_csn
+            self.d_flash = torch.from_numpy(fs.view(np.int32)).to(dev)
+            self.d_pins = torch.from_numpy(flash_pins.words().view(np.int32)).to(dev)
+            img = np.frombuffer(bytes(flash_image), np.uint8) if flash_image else np.zeros(1, np.uint8)
+            self.d_image = torch.from_numpy(img.copy()).to(dev)
+            self.image_size = len(flash_image)
+        self._ops_reset_val = None
+
+    def _upload_ops(self, reset_val):
+        if reset_val == self._ops_reset_val:
+            return
+        ops, start = [], [0]
+        for k in range(len(self.sched)):
+            ops += self.sched.edge_ops(k, self.domains, self.reset_pos, reset_val,
+                                       self.constants, self.model_driven)
+            start.append(len(ops))
+        arr = np.asarray(ops if ops else [(NC, 0)], np.uint32).reshape(-1, 2)
+        self.ops = arr
+        self.op_start = np.asarray(start, np.uint32)
+        self.d_ops = self.torch.from_numpy(arr.reshape(-1).view(np.int32)).to("cuda")
+        self.d_start = self.torch.from_numpy(self.op_start.view(np.int32)).to("cuda")
+        self._ops_reset_val = reset_val
+
+    def run(self, num_edges, batch=1024, trace=False):
+        """Run num_edges edges; returns the trace (uint32[num_edges, 2*ss])
+        when asked."""
+        torch = self.torch
+        out = []
+        done = 0
+        while done < num_edges:
+            n = min(batch, num_edges - done)
+            in_reset = self.edge < self.reset_edges
+            if in_reset and self.edge + n > self.reset_edges:
+                n = self.reset_edges - self.edge
+            reset_val = self.reset_active if in_reset else 1 - self.reset_active
+            self._upload_ops(reset_val)
+            if self.flash:
+                self.d_flash.view(torch.uint8)[_IN_RESET_OFFSET] = 1 if in_reset else 0
+            tr = torch.zeros((n, 2 * self.ss), dtype=torch.int32, device="cuda") if trace else None
+            N.check(N.hip().jq_cosim_edges(
+                self.script.num_blocks, self.script.num_major_stages, self.d_bs.data_ptr(),
+                self.d_bd.data_ptr(), self.d_sram.data_ptr(), self.ss, self.states.data_ptr(),
+                self.d_ops.data_ptr(), self.d_start.data_ptr(), len(self.sched),
+                self.edge % len(self.sched), n,
+                self.d_flash.data_ptr() if self.flash else None,
+                self.d_pins.data_ptr() if self.flash else None,
+                self.d_image.data_ptr() if self.flash else None,
+                self.image_size if self.flash else 0,
+                tr.data_ptr() if tr is not None else None,
+                ctypes.c_void_p(self.stream.cuda_stream)))
+            if tr is not None:
+                out.append(tr)
+            self.edge += n
+            done += n
+        self.stream.synchronize()
+        from .sim import device_error
+        if device_error():
+            raise RuntimeError("device barrier timeout during co-simulation")
+        if trace:
+            return torch.cat(out).cpu().numpy().view(np.uint32) if out else \
+                np.zeros((0, 2 * self.ss), np.uint32)
+        return None
+
+    def flash_state(self):
+        """The device FlashState as 48 bytes."""
+        return self.d_flash.cpu().numpy().view(np.uint8).copy() if self.flash else None
+
+    def snapshots(self):
+        return self.states.cpu().numpy().view(np.uint32).copy()

@@ -20,6 +20,9 @@ EVENT_DTYPE = np.dtype([("event_type", "<u4"), ("message_id", "<u4"),
 def build():
     import subprocess
     subprocess.check_call(["make", "-s", "-C", HERE])
+    # the reference's SPI flash model, where the reference tree is present
+    if os.path.exists("/root/reference/csrc/spiflash_model.cc"):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
 
 
 def lib():

@@ -1,0 +1,233 @@
+"""Co-simulation host logic and its CPU oracle (no GPU).
+
+* The flash restatement (oracle/cosim_oracle.py) against the reference's own
+  SPI flash model compiled from csrc/spiflash_model.cc into oracle/_ref,
+  step by step over SPI transactions of every command class.
+* The multi-clock scheduler (src/sim/cosim_metal.rs:1433-1508) against
+  closed forms: alternating edges, edge counts per common period, phase
+  offsets.
+* The edge loop end to end: a gate-level SPI master reads two bytes of a
+  random flash image through the loop; the bytes it captures must be the
+  image's (a data-dependent check of the loop order, the setup delay and
+  the data-in path).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "oracle"))
+sys.path.insert(0, HERE)
+
+import cosim_oracle as co  # noqa: E402
+from cosim_design import spi_master_verilog  # noqa: E402
+from jacquard_amd import cosim  # noqa: E402
+
+
+def _spi_sequence(rng, n_tx):
+    """(clk, csn, d_o) steps of n_tx transactions: a command byte (one of
+    every class), address and data bits, clock glitches while deselected,
+    random lengths."""
+    steps = []
+    for _ in range(n_tx):
+        cmd = int(rng.choice([0x03, 0xeb, 0x9f, 0xab, 0x05, 0x77, int(rng.integers(0, 256))]))
+        steps.append((0, 1, int(rng.integers(0, 16))))
+        steps.append((0, 0, 0))
+        nbits = 8 + int(rng.integers(0, 80))
+        byte = cmd
+        for b in range(nbits):
+            if b < 8:
+                bit = (byte >> (7 - b)) & 1
+                d = bit
+            else:
+                d = int(rng.integers(0, 16))
+            steps.append((0, 0, d))
+            steps.append((1, 0, d))
+            if rng.random() < 0.05:  # a repeated level (no edge)
+                steps.append((1, 0, d))
+        steps.append((0, 0, 0))
+        steps.append((0, 1, 0))
+        if rng.random() < 0.3:  # clocks while deselected
+            steps += [(1, 1, 0), (0, 1, 0)]
+    return steps
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_flash_restatement_matches_reference_model(seed):
+    if co.RefFlash.lib() is None:
+        pytest.skip("oracle/_ref/libspiflash_ref.so not built (no reference tree here)")
+    rng = np.random.default_rng(seed)
+    size = int(rng.choice([64, 4096]))
+    img = bytes(rng.integers(0, 256, size, dtype=np.uint8))
+    ref = co.RefFlash(img)
+    mine = co.Flash(img, cosim_init=False)
+    seq = _spi_sequence(rng, 12)
+    for i, (clk, csn, d) in enumerate(seq):
+        got = mine.step_model(clk, csn, d)
+        exp = ref.step(clk, csn, d)
+        assert got == exp, f"step {i}: d_i {got:#x} != {exp:#x}"
+        assert mine.command == ref.command() and mine.byte_count == ref.byte_count(), i
+
+
+def test_flash_read_commands_return_image_bytes():
+    # closed forms: a single read returns image[a], image[a+1], ...; past
+    # the end 0xFF; read-ID returns CA 7C A7 FF
+    img = bytes(range(256)) * 2
+    f = co.Flash(img, cosim_init=False)
+
+    def clock_byte(v, width=1):
+        # the controller samples MISO before each rising clock: d_i changed
+        # on the falling clock before it
+        out = 0
+        for k in range(8 // width):
+            if width == 1:
+                d = (v >> (7 - k)) & 1
+                out = (out << 1) | ((f.d_i >> 1) & 1)
+            else:
+                d = (v >> (4 - 4 * k)) & 0xF
+                out = (out << 4) | (f.d_i & 0xF)
+            f.step_model(1, 0, d)
+            f.step_model(0, 0, d)
+        return out
+
+    def read(cmd, addr, n, width=1, dummies=0):
+        f.step_model(0, 1, 0)
+        f.step_model(0, 0, 0)
+        clock_byte(cmd)
+        if addr is not None:
+            for sh in (16, 8, 0):
+                clock_byte((addr >> sh) & 0xFF, width)
+        for _ in range(dummies):
+            clock_byte(0, width)
+        got = [clock_byte(0, width) for _ in range(n)]
+        f.step_model(0, 1, 0)
+        return got
+
+    assert read(0x03, 0x10, 3) == [img[0x10], img[0x11], img[0x12]]
+    assert read(0x03, 0x1FF, 2) == [img[0x1FF], 0xFF]
+    # quad read: address in 4-bit mode, a mode byte and two dummy bytes
+    assert read(0xeb, 0x20, 3, width=4, dummies=3) == [img[0x20], img[0x21], img[0x22]]
+    assert read(0x9f, None, 5) == [0xCA, 0x7C, 0xA7, 0xFF, 0xCA]
+    assert f.last_error_cmd == 0
+    read(0x42, None, 1)
+    assert f.last_error_cmd == 0x42
+
+
+def test_scheduler_single_clock():
+    s = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
+    assert s.gcd_ps == 5000
+    assert s.schedule == [[(True, False)], [(False, True)]]
+
+
+@pytest.mark.parametrize("hps,offsets", [((10, 15), (0, 0)), ((6, 4), (0, 2)),
+                                         ((7, 7, 14), (0, 7, 0))])
+def test_scheduler_multi_clock_closed_forms(hps, offsets):
+    timings = [cosim.ClockTiming(h, o, i) for i, (h, o) in enumerate(zip(hps, offsets))]
+    s = cosim.MultiClockScheduler(timings)
+    g = 0
+    for h in hps:
+        g = np.gcd(g, h)
+    for o in offsets:
+        if o:
+            g = np.gcd(g, o)
+    period = int(np.lcm.reduce([2 * h for h in hps]))
+    assert s.gcd_ps == g and len(s) == period // g
+    for i, (h, o) in enumerate(zip(hps, offsets)):
+        ticks = [k for k in range(len(s)) if any(s.schedule[k][i])]
+        # edges every h ps from the offset, alternating falling / rising
+        assert ticks == [k for k in range(len(s)) if k * g >= o and (k * g - o) % h == 0]
+        kinds = ["f" if s.schedule[k][i][0] else "r" for k in ticks]
+        assert kinds == ["f" if j % 2 == 0 else "r" for j in range(len(ticks))]
+        assert not any(a and b for a, b in (s.schedule[k][i] for k in range(len(s))))
+
+
+def test_scheduler_rejects_incommensurable_periods():
+    with pytest.raises(ValueError):
+        cosim.MultiClockScheduler([cosim.ClockTiming(1_000_003), cosim.ClockTiming(999_983)])
+    with pytest.raises(ValueError):
+        cosim.MultiClockScheduler([])
+
+
+def test_edge_ops_order_and_values():
+    s = cosim.MultiClockScheduler([cosim.ClockTiming(10, 0, 0), cosim.ClockTiming(5, 0, 1)])
+    doms = [cosim.ClockDomain(3, [40], [41]), cosim.ClockDomain(None, [50], [])]
+    # tick 0: both fall; tick 1: domain 1 rises
+    assert s.edge_ops(0, doms, reset_pos=7, reset_val=1, constants=[(9, 1)],
+                      model_driven=[11]) == [(7, 1), (3, 0), (40, 0), (41, 1), (50, 0),
+                                             (9, 1), (11, 0)]
+    assert s.edge_ops(1, doms) == [(40, 0), (41, 0), (50, 1)]
+    assert s.edge_ops(2, doms) == [(3, 1), (40, 1), (41, 0), (50, 0)]
+
+
+def spi_setup(address, image, reset_edges=8):
+    """(script, pins words, ops builder, output bits) of the SPI master."""
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    nl = Netlist.from_verilog(text=spi_master_verilog(address))
+    s = Compiled(nl).flatten(1)
+    ins, outs = nl.ports()
+    inbit = {n: s.input_map[p] for n, p, _ in ins}
+    outbit = {n: s.output_map[lit] for n, lit in outs if lit > 1}
+    pins = cosim.FlashPins(clk_out=outbit["flash_clk"], csn_out=outbit["flash_csn"],
+                           d_out=(outbit["flash_d0"], cosim.NC, cosim.NC, cosim.NC),
+                           d_in=(cosim.NC, inbit["miso"], cosim.NC, cosim.NC))
+    doms = [d for _, d in cosim.clock_domains_of(s)]
+    return s, pins, doms, inbit, outbit
+
+
+def oracle_edges(s, pins, doms, rst_pos, image, reset_edges, num_edges, sched):
+    """The CPU edge loop with the reference's reset batching."""
+    ss = s.effective_state_size()
+    st = np.zeros(2 * ss, np.uint32)
+    fl = co.Flash(image)
+    traces = []
+    done = 0
+    for in_reset, n in ((1, min(reset_edges, num_edges)), (0, num_edges - min(reset_edges, num_edges))):
+        if n == 0:
+            continue
+        ops, start = [], [0]
+        for k in range(len(sched)):
+            ops += sched.edge_ops(k, doms, rst_pos, in_reset)
+            start.append(len(ops))
+        fl.in_reset = in_reset
+        traces.append(co.run_edges(s, st, np.asarray(ops, np.uint32).reshape(-1, 2),
+                                   np.asarray(start, np.uint32), done % len(sched), n, fl,
+                                   pins.words()))
+        done += n
+    return np.concatenate(traces), st, fl
+
+
+@pytest.mark.parametrize("address", [0x000123, 0x000FFF, 0x000040])
+def test_spi_master_reads_image_through_the_oracle_loop(address):
+    rng = np.random.default_rng(address)
+    image = bytes(rng.integers(0, 256, 4096, dtype=np.uint8))
+    s, pins, doms, inbit, outbit = spi_setup(address, image)
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
+    _, st, fl = oracle_edges(s, pins, doms, inbit["rst"], image, 8, 300, sched)
+    ss = s.effective_state_size()
+    out = st[ss:]
+    rx = sum(co._bit(out, outbit[f"rx_o{i}"]) << i for i in range(16))
+    hi = image[address]
+    lo = image[address + 1] if address + 1 < len(image) else 0xFF
+    assert rx == (hi << 8 | lo)
+    assert fl.command == 0x03 and fl.last_error_cmd == 0
+
+
+def test_flash_held_in_reset_reads_nothing():
+    # with the flash in reset throughout, MISO stays high: rx is all ones
+    image = bytes(4096)
+    s, pins, doms, inbit, outbit = spi_setup(0x10, image)
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
+    ss = s.effective_state_size()
+    st = np.zeros(2 * ss, np.uint32)
+    fl = co.Flash(image)
+    ops, start = [], [0]
+    for k in range(len(sched)):
+        ops += sched.edge_ops(k, doms, inbit["rst"], 0)
+        start.append(len(ops))
+    co.run_edges(s, st, np.asarray(ops, np.uint32).reshape(-1, 2),
+                 np.asarray(start, np.uint32), 0, 300, fl, pins.words())
+    rx = sum(co._bit(st[ss:], outbit[f"rx_o{i}"]) << i for i in range(16))
+    assert rx == 0xFFFF and fl.byte_count == 0

@@ -1,0 +1,77 @@
+"""Device co-simulation edges (jq_cosim_edges) against the CPU edge loop of
+oracle/cosim_oracle.py: every snapshot of every edge and the final flash
+state, bit for bit."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "oracle"))
+sys.path.insert(0, HERE)
+
+import cosim_oracle as co  # noqa: E402
+from helpers import CONFIG1, design  # noqa: E402
+from jacquard_amd import cosim  # noqa: E402
+from test_cosim import oracle_edges, spi_setup  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("address,batch", [(0x000123, 64), (0x000FFF, 1000)])
+def test_spi_master_cosim_matches_oracle(address, batch):
+    rng = np.random.default_rng(address)
+    image = bytes(rng.integers(0, 256, 4096, dtype=np.uint8))
+    s, pins, doms, inbit, outbit = spi_setup(address, image)
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
+    edges, reset = 300, 8
+    print(f"cosim spi master: address {address:#x}, {edges} edges in batches of {batch}",
+          flush=True)
+    c = cosim.Cosim(s, sched, doms, reset_pos=inbit["rst"], reset_active=1,
+                    reset_edges=reset, flash_pins=pins, flash_image=image)
+    got = c.run(edges, batch=batch, trace=True)
+    exp, st, fl = oracle_edges(s, pins, doms, inbit["rst"], image, reset, edges, sched)
+    assert got.shape == exp.shape
+    bad = np.nonzero((got != exp).any(axis=1))[0]
+    assert bad.size == 0, f"first differing edge {bad[:1]}"
+    assert np.array_equal(c.snapshots(), st)
+    assert bytes(c.flash_state()) == fl.to_record().tobytes()
+    ss = s.effective_state_size()
+    rx = sum(co._bit(st[ss:], outbit[f"rx_o{i}"]) << i for i in range(16))
+    lo = image[address + 1] if address + 1 < len(image) else 0xFF
+    assert rx == (image[address] << 8 | lo)
+
+
+def test_cosim_random_netlist_two_clock_domains():
+    # a multi-block script, a second clock domain whose flags land on data
+    # inputs, constant and model-driven inputs; no flash
+    nl, comp = design("cfg1", level_split=6, **CONFIG1)
+    s = comp.flatten(8)
+    assert s.num_major_stages > 1
+    ins, _ = nl.ports()
+    data_bits = [s.input_map[p] for (_, p, f) in ins if not f and p in s.input_map]
+    doms = [d for _, d in cosim.clock_domains_of(s)]
+    doms.append(cosim.ClockDomain(data_bits[0], [data_bits[1]], [data_bits[2]]))
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000, 0, 0),
+                                       cosim.ClockTiming(7500, 2500, 1)])
+    consts = [(data_bits[3], 1), (data_bits[4], 0), (data_bits[5], 1)]
+    driven = [data_bits[6]]
+    edges = 96
+    print(f"cosim random netlist: {s.num_blocks} blocks, {len(sched)} schedule edges, "
+          f"{edges} edges", flush=True)
+    c = cosim.Cosim(s, sched, doms, constants=consts, model_driven=driven)
+    got = c.run(edges, batch=40, trace=True)
+    ops, start = [], [0]
+    for k in range(len(sched)):
+        ops += sched.edge_ops(k, doms, None, 0, consts, driven)
+        start.append(len(ops))
+    st = np.zeros(2 * s.effective_state_size(), np.uint32)
+    exp = co.run_edges(s, st, np.asarray(ops, np.uint32).reshape(-1, 2),
+                       np.asarray(start, np.uint32), 0, edges)
+    bad = np.nonzero((got != exp).any(axis=1))[0]
+    assert bad.size == 0, f"first differing edge {bad[:1]}"
+    # and without a trace: the same final snapshots
+    c2 = cosim.Cosim(s, sched, doms, constants=consts, model_driven=driven)
+    assert c2.run(edges, batch=edges) is None
+    assert np.array_equal(c2.snapshots(), st)
