@@ -48,7 +48,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cycles-per-step", type=int, default=10_000)
-    ap.add_argument("--workload", choices=["plain", "xtiming", "lanes4m", "mult16m"],
+    ap.add_argument("--workload", choices=["plain", "xtiming", "lanes4m", "mult16m",
+                                           "cosim"],
                     default="plain",
                     help="plain: the 1M-gate 2-state drop-in (the headline line); "
                          "xtiming: the 4M-gate netlist with 8 SRAMs, 10%% initial-X "
@@ -56,7 +57,9 @@ def parse():
                          "lanes4m: batched+learned lanes vs the plain drop-in on that "
                          "4M-gate netlist (value lanes); mult16m: the 16M-gate structured "
                          "netlist (64 16x16 multiplier tiles + 2M-gate control AIG, 1M "
-                         "DFF), --lanes lanes of --cycles cycles, learned vs heuristic")
+                         "DFF), --lanes lanes of --cycles cycles, learned vs heuristic; "
+                         "cosim: reactive co-simulation edges (SPI master + flash model, "
+                         "and the 1M-gate netlist clocked edge by edge)")
     ap.add_argument("--gates", type=int, default=None)
     ap.add_argument("--depth", type=int, default=None)
     ap.add_argument("--dffs", type=int, default=None)
@@ -559,6 +562,132 @@ def bench_mult16m(a, torch):
     }), flush=True)
 
 
+def bench_cosim(a, torch):
+    """Reactive co-simulation edges per second.  (1) The SPI master with the
+    flash model: 20,000 edges in batches of 1024, the first 400 edges checked
+    against the CPU edge loop and the two bytes it reads against the image.
+    (2) The default 1M-gate netlist clocked edge by edge (one clock domain,
+    no flash): 400 edges, the first 4 checked against the CPU loop.  Each edge
+    is one between-edges launch and one scan-kernel cycle."""
+    import numpy as np
+    from jacquard_amd import cosim
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cosim_oracle as co
+    res = {}
+    # (1) SPI master + flash
+    address = 0x000123
+    rng = np.random.default_rng(a.seed)
+    image = bytes(rng.integers(0, 256, 1 << 16, dtype=np.uint8))
+    nl = Netlist.from_verilog(text=cosim.spi_master_verilog(address))
+    s = Compiled(nl).flatten(1)
+    ins, outs = nl.ports()
+    inbit = {n: s.input_map[p] for n, p, _ in ins}
+    outbit = {n: s.output_map[lit] for n, lit in outs if lit > 1}
+    pins = cosim.FlashPins(clk_out=outbit["flash_clk"], csn_out=outbit["flash_csn"],
+                           d_out=(outbit["flash_d0"], cosim.NC, cosim.NC, cosim.NC),
+                           d_in=(cosim.NC, inbit["miso"], cosim.NC, cosim.NC))
+    doms = [d for _, d in cosim.clock_domains_of(s)]
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
+    reset, check, edges = 8, 400, 20_000
+
+    def make():
+        return cosim.Cosim(s, sched, doms, reset_pos=inbit["rst"], reset_active=1,
+                           reset_edges=reset, flash_pins=pins, flash_image=image)
+    c = make()
+    got = c.run(check, batch=1024, trace=True)
+    st = np.zeros(2 * s.effective_state_size(), np.uint32)
+    fl = co.Flash(image)
+    tr = []
+    for in_reset, n, off in ((1, reset, 0), (0, check - reset, reset)):
+        ops, start = [], [0]
+        for k in range(len(sched)):
+            ops += sched.edge_ops(k, doms, inbit["rst"], in_reset)
+            start.append(len(ops))
+        fl.in_reset = in_reset
+        tr.append(co.run_edges(s, st, np.asarray(ops, np.uint32).reshape(-1, 2),
+                               np.asarray(start, np.uint32), off % len(sched), n, fl,
+                               pins.words()))
+    if not np.array_equal(got, np.concatenate(tr)):
+        raise AssertionError("co-simulation edges differ from the CPU edge loop")
+    ss = s.effective_state_size()
+    rx = sum(co._bit(st[ss:], outbit[f"rx_o{i}"]) << i for i in range(16))
+    if rx != (image[address] << 8 | image[address + 1]):
+        raise AssertionError("the SPI master read the wrong bytes")
+    c = make()
+    c.run(64, batch=64)  # warm-up
+    c = make()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    c.run(edges, batch=1024)
+    wall = time.perf_counter() - t
+    t = time.perf_counter()
+    fl_cpu = co.Flash(image)
+    st = np.zeros(2 * ss, np.uint32)
+    ops, start = [], [0]
+    for k in range(len(sched)):
+        ops += sched.edge_ops(k, doms, inbit["rst"], 0)
+        start.append(len(ops))
+    fl_cpu.in_reset = 0
+    n_cpu = 2000
+    co.run_edges(s, st, np.asarray(ops, np.uint32).reshape(-1, 2),
+                 np.asarray(start, np.uint32), 0, n_cpu, fl_cpu, pins.words())
+    cpu = n_cpu / (time.perf_counter() - t)
+    res["spi_flash"] = {"edges": edges, "edges_per_s": edges / wall,
+                        "us_per_edge": wall / edges * 1e6, "verified_edges": check,
+                        "gates": nl.counts()["num_and"], "state_words": ss,
+                        "cpu_edges_per_s": cpu, "cpu_edges": n_cpu}
+    log(f"cosim spi+flash: {edges / wall:,.0f} edges/s ({wall / edges * 1e6:.1f} us per edge), "
+        f"CPU loop {cpu:,.0f} edges/s")
+    # (2) the 1M-gate netlist clocked edge by edge
+    import ctypes
+    from jacquard_amd import _native as N
+    nl2, comp = build_design(a)
+    r = ctypes.c_int(0)
+    N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(r)))
+    s2 = comp.flatten(min(max(comp.parts_per_stage()), r.value))
+    doms2 = [d for _, d in cosim.clock_domains_of(s2)]
+    c2 = cosim.Cosim(s2, sched, doms2)
+    got = c2.run(4, batch=4, trace=True)
+    ops, start = [], [0]
+    for k in range(len(sched)):
+        ops += sched.edge_ops(k, doms2)
+        start.append(len(ops))
+    st2 = np.zeros(2 * s2.effective_state_size(), np.uint32)
+    t = time.perf_counter()
+    exp = co.run_edges(s2, st2, np.asarray(ops, np.uint32).reshape(-1, 2),
+                       np.asarray(start, np.uint32), 0, 4)
+    cpu2 = 4 / (time.perf_counter() - t)
+    if not np.array_equal(got, exp):
+        raise AssertionError("1M-gate co-simulation edges differ from the CPU edge loop")
+    e2 = 400
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    c2.run(e2, batch=e2)
+    wall2 = time.perf_counter() - t
+    res["netlist_1m"] = {"edges": e2, "edges_per_s": e2 / wall2,
+                         "us_per_edge": wall2 / e2 * 1e6, "verified_edges": 4,
+                         "blocks": s2.num_blocks, "major_stages": s2.num_major_stages,
+                         "cpu_edges_per_s": cpu2, "cpu_edges": 4}
+    log(f"cosim 1M gates: {e2 / wall2:,.0f} edges/s ({wall2 / e2 * 1e6:.1f} us per edge)")
+    print(json.dumps({
+        "metric": "co-simulation edges per second", "value": edges / wall,
+        "unit": "edges/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"reactive co-simulation: gate-level SPI master "
+                               f"({nl.counts()['num_and']} AND, 23 DFF) reading a 64 KiB "
+                               f"flash image through the device SPI flash model, one clock "
+                               f"domain, {edges} edges in batches of 1024; and the 1M-gate "
+                               f"random AIG clocked edge by edge",
+                   "forms": res},
+        "cpu_baseline": {"value": cpu, "unit": "edges/s", "cores": 1, "kind": "port",
+                         "sample": f"the CPU edge loop (oracle/cosim_oracle.py over the "
+                                   f"C interpreter) for {n_cpu} edges of the SPI master"},
+    }), flush=True)
+
+
 def cpu_baseline(a, script):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -740,6 +869,8 @@ def main():
         return bench_lanes4m(a, torch)
     if a.workload == "mult16m":
         return bench_mult16m(a, torch)
+    if a.workload == "cosim":
+        return bench_cosim(a, torch)
     if a.workload == "xtiming":
         x = bench_xtiming(a, torch)
         s = x["script"]

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "lib")
 INC = os.path.join(ROOT, "include")
 
 DEVICE_SOURCES = ["jq_scan.hip", "jq_batched.hip", "jq_mlp.hip", "jq_vcd.hip",
-                  "jq_probe.hip", "jq_cosim.hip"]
+                  "jq_probe.hip"]
 HOST_SOURCES = ["aig.cpp", "netgen.cpp", "compile.cpp", "verilog.cpp", "capi.cpp", "vcd.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,24 +1,22 @@
-// Reactive co-simulation on the device: the work between two simulated clock
-// edges of the reference's Metal co-simulation (state preparation, SPI flash
-// data-in and the SPI flash model, csrc/kernel_v1.metal:654-1025), driven
-// edge by edge as src/sim/cosim_metal.rs:547-651 does:
+// Device side of the reactive co-simulation: the work between two simulated
+// clock edges of the reference's Metal co-simulation (state preparation,
+// SPI flash data-in and the SPI flash model, csrc/kernel_v1.metal:654-1025),
+// in the edge order of src/sim/cosim_metal.rs:574-650:
 //
 //   per edge e:  prepare(e) -> flash data-in -> simulate one cycle
 //                -> flash model step (sees the outputs of edge e)
 //
 // The state buffer holds two snapshots: the input of the edge at [0, ss) and
-// its output at [ss, 2ss).  The flash step of edge e and the preparation of
-// edge e+1 run in one single-workgroup launch (`between_kernel`), so an edge
-// costs two launches: that kernel and one cycle of the scan kernel.  Nothing
-// returns to the host between edges.
-#include <hip/hip_runtime.h>
-
-#include <string>
+// its output at [ss, 2ss).  `between_phase` runs in one workgroup between
+// two cycles of the persistent co-simulation kernel (jq_scan.hip); other
+// workgroups read and write the snapshots, so every snapshot access here is
+// agent-coherent.
+#pragma once
 
 #include "jq_common.hpp"
 
 namespace jqd {
-namespace {
+namespace cosim {
 
 // The reference's FlashState (csrc/kernel_v1.metal:731-749,
 // src/sim/cosim_metal.rs:90-111), 48 bytes, persistent across edges.
@@ -37,33 +35,30 @@ static_assert(sizeof(FlashState) == 48, "FlashState layout");
 // four data inputs d0..d3 (0xFFFFFFFF: not connected), of the SPI clock and
 // chip-select outputs and of the four data outputs, then has_flash.
 
-struct BetweenArgs {
+struct CosimArgs {
   u32 *states;
   u32 ss;
   const u32 *ops;        // (position, value) pairs of every schedule edge
   const u32 *op_start;   // CSR over schedule edges
-  u32 prep_edge;         // schedule edge to prepare, ~0u: none
-  u32 flash_step;        // 1: run the flash step on the current outputs first
+  u32 num_sched, sched_offset, num_edges;
   FlashState *fs;        // null: no flash
   const u32 *pins;       // 11 pin words
   const u8 *flash_data;
   u32 flash_size;
-  u32 *trace;            // null, or 2*ss words of this edge's snapshots
+  u32 *trace;            // null, or num_edges x 2*ss words
 };
 
 // Bit access inside one snapshot of ss words; positions past its end read
 // as 0 and are not written.
 __device__ __forceinline__ u32 bit_at(const u32 *s, u32 ss, u32 pos) {
-  return (pos >> 5) < ss ? (s[pos >> 5] >> (pos & 31u)) & 1u : 0u;
+  return (pos >> 5) < ss ? (ld_coherent(s + (pos >> 5)) >> (pos & 31u)) & 1u : 0u;
 }
 
 __device__ __forceinline__ void put_bit(u32 *s, u32 ss, u32 pos, u32 v) {
   if ((pos >> 5) >= ss) return;
   const u32 m = 1u << (pos & 31u);
-  if (v)
-    s[pos >> 5] |= m;
-  else
-    s[pos >> 5] &= ~m;
+  const u32 w = ld_coherent(s + (pos >> 5));
+  st_coherent(s + (pos >> 5), v ? (w | m) : (w & ~m));
 }
 
 // A byte completed on the SPI bus (csrc/spiflash_model.cc:40-101; the Metal
@@ -139,7 +134,7 @@ __device__ void flash_eval(FlashState &f, u32 clk, u32 csn, u32 d_out, u32 &mode
 // system clock edge).  The reference evaluates twice, with the previous and
 // then the current data; the second evaluation sees no clock or chip-select
 // edge and changes nothing, so one evaluation is made here.
-__device__ void flash_step(const BetweenArgs &A) {
+__device__ void flash_step(const CosimArgs &A) {
   FlashState f = *A.fs;
   const u32 *out = A.states + A.ss;
   const u32 clk = bit_at(out, A.ss, A.pins[4]);
@@ -160,21 +155,50 @@ __device__ void flash_step(const BetweenArgs &A) {
   *A.fs = f;
 }
 
-__global__ __launch_bounds__(256) void between_kernel(BetweenArgs A) {
-  const u32 t = threadIdx.x;
-  const bool flash = A.fs != nullptr && A.pins[10] != 0;
-  if (A.flash_step && flash && t == 0) flash_step(A);
-  if (A.trace) {
-    for (u32 i = t; i < 2 * A.ss; i += 256) A.trace[i] = A.states[i];
+// dst[i] = src[i] for i < n over one workgroup, src read agent-coherently
+// (dst written so when `coherent`), with kCopyBatch loads in flight per
+// thread before their stores.
+constexpr u32 kCopyBatch = 8;
+__device__ __forceinline__ void copy_words(u32 *dst, const u32 *src, u32 n, u32 t,
+                                           bool coherent) {
+  for (u32 base = t; base < n; base += 256 * kCopyBatch) {
+    u32 v[kCopyBatch];
+#pragma unroll
+    for (u32 k = 0; k < kCopyBatch; ++k) {
+      const u32 i = base + 256 * k;
+      v[k] = i < n ? ld_coherent(src + i) : 0u;
+    }
+#pragma unroll
+    for (u32 k = 0; k < kCopyBatch; ++k) {
+      const u32 i = base + 256 * k;
+      if (i < n) {
+        if (coherent)
+          st_coherent(dst + i, v[k]);
+        else
+          dst[i] = v[k];
+      }
+    }
   }
-  if (A.prep_edge == ~0u) return;
+}
+
+// Between cycle e-1 and cycle e, in one workgroup (t = thread index):
+// the flash step on the outputs of edge e-1 and its trace copy (when
+// flash_step), then the preparation of edge e (prep_edge != ~0u).  The
+// caller waits for this workgroup's stores before the grid barrier.
+__device__ void between_phase(const CosimArgs &A, u32 t, bool step, u32 prep_edge,
+                              u32 *trace) {
+  const bool flash = A.fs != nullptr && A.pins[10] != 0;
+  if (step && flash && t == 0) flash_step(A);
+  if (trace) copy_words(trace, A.states, 2 * A.ss, t, false);
+  if (prep_edge == ~0u) return;
   __syncthreads();
   // state preparation (csrc/kernel_v1.metal:678-710): the output snapshot
   // becomes the next input, then the edge's bit operations in order
-  for (u32 i = t; i < A.ss; i += 256) A.states[i] = A.states[A.ss + i];
+  copy_words(A.states, A.states + A.ss, A.ss, t, true);
+  wait_vmem();
   __syncthreads();
   if (t == 0) {
-    const u32 b = A.op_start[A.prep_edge], e = A.op_start[A.prep_edge + 1];
+    const u32 b = A.op_start[prep_edge], e = A.op_start[prep_edge + 1];
     for (u32 k = b; k < e; ++k) put_bit(A.states, A.ss, A.ops[2 * k], A.ops[2 * k + 1]);
     // flash data-in (csrc/kernel_v1.metal:896-918): the model's d_i
     if (flash) {
@@ -184,57 +208,5 @@ __global__ __launch_bounds__(256) void between_kernel(BetweenArgs A) {
   }
 }
 
-int launch_between(const BetweenArgs &A, hipStream_t stream) {
-  hipLaunchKernelGGL(between_kernel, dim3(1), dim3(256), 0, stream, A);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return set_error((std::string("cosim launch failed: ") + hipGetErrorString(e)).c_str());
-  return 0;
-}
-
-}  // namespace
+}  // namespace cosim
 }  // namespace jqd
-
-using namespace jqd;
-
-extern "C" size_t jq_cosim_flash_state_bytes(void) { return sizeof(FlashState); }
-
-extern "C" int jq_cosim_edges(usize num_blocks, usize num_major_stages,
-                              const usize *blocks_start, const u32 *blocks_data,
-                              u32 *sram_data, usize state_size, u32 *states,
-                              const u32 *edge_ops, const u32 *edge_op_start,
-                              u32 num_sched, u32 sched_offset, u32 num_edges,
-                              void *flash_state, const u32 *flash_pins,
-                              const u8 *flash_data, u32 flash_data_size, u32 *trace,
-                              void *stream) {
-  set_error("");
-  if (num_edges == 0) return 0;
-  if (!states || state_size == 0 || state_size > 0xFFFFFFFFull / 2)
-    return set_error("cosim: state buffer missing or state_size out of range");
-  if (num_sched == 0 || !edge_op_start || !edge_ops)
-    return set_error("cosim: empty edge schedule");
-  if (flash_state && !flash_pins) return set_error("cosim: flash state without flash pins");
-  if (flash_data_size && !flash_data) return set_error("cosim: flash image missing");
-  hipStream_t s = (hipStream_t)stream;
-  BetweenArgs A{};
-  A.states = states;
-  A.ss = (u32)state_size;
-  A.ops = edge_ops;
-  A.op_start = edge_op_start;
-  A.fs = reinterpret_cast<FlashState *>(flash_state);
-  A.pins = flash_pins;
-  A.flash_data = flash_data;
-  A.flash_size = flash_data_size;
-  for (u32 e = 0; e < num_edges; ++e) {
-    A.flash_step = e > 0;
-    A.trace = (e > 0 && trace) ? trace + (size_t)(e - 1) * 2 * state_size : nullptr;
-    A.prep_edge = (u32)(((usize)sched_offset + e) % num_sched);
-    if (launch_between(A, s)) return -1;
-    if (jq_simulate_scan(num_blocks, num_major_stages, blocks_start, blocks_data, sram_data,
-                         sram_data, 1, state_size, states, nullptr, nullptr, 0, 0, stream))
-      return -1;
-  }
-  A.flash_step = 1;
-  A.trace = trace ? trace + (size_t)(num_edges - 1) * 2 * state_size : nullptr;
-  A.prep_edge = ~0u;
-  return launch_between(A, s);
-}

@@ -27,6 +27,7 @@
 #include <string>
 
 #include "jq_common.hpp"
+#include "jq_cosim.hpp"
 
 namespace jqd {
 
@@ -585,6 +586,67 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
     }
   }
 }
+// Persistent co-simulation: C.num_edges edges of one cycle each over the
+// two snapshots of C.states, workgroup 0 doing the between-edges work of
+// jq_cosim.hpp ahead of every cycle (and once after the last).  A cycle
+// walks the partitions as scan_kernel does, except that the first
+// partition of a cycle is loaded after the barrier that follows the
+// preparation, which may change any input word.
+__global__ __launch_bounds__(256) void cosim_kernel(ScanArgs A, cosim::CosimArgs C) {
+  __shared__ u32 st_v[256], st_x[256], hin_v[256], hin_x[256];
+  __shared__ u32 wo_v[256], wo_x[256];
+  __shared__ u16 st_arr[256], wo_arr[256];
+  const int t = threadIdx.x;
+  st_x[t] = 0;
+  hin_x[t] = 0;
+  wo_x[t] = 0;
+  wo_arr[t] = 0;
+  st_arr[t] = 0;
+  u32 gen = read_gen(A.bar);
+  __syncthreads();
+  const u32 b = blockIdx.x;
+  const u32 nb = A.num_blocks, M = A.num_major_stages;
+  const u32 *bd = A.blocks_data;
+  const u32 *in = A.states;
+  u32 *out = A.states + A.state_size;
+  for (u32 e = 0; e <= C.num_edges; ++e) {
+    if (b == 0) {
+      u32 *tr = (e > 0 && C.trace) ? C.trace + (size_t)(e - 1) * 2 * C.ss : nullptr;
+      const u32 prep =
+          e < C.num_edges ? (u32)(((usize)C.sched_offset + e) % C.num_sched) : ~0u;
+      cosim::between_phase(C, (u32)t, e > 0, prep, tr);
+    }
+    if (e == C.num_edges) break;
+    grid_sync_coherent(A.bar, nb, b, gen, &g_error);
+    PartHdr h = M ? load_hdr(bd, A.blocks_start[b]) : no_hdr();
+    PartPre P;
+    if (h.num_stages) load_pre(P, bd, h, in);
+    for (u32 m = 0; m < M; ++m) {
+      const bool wrap = m + 1 == M;
+      for (;;) {
+        const bool last = h.num_stages == 0 || h.is_last != 0;
+        PartHdr hn = no_hdr();
+        if (!last)
+          hn = load_hdr(bd, h.base + part_words(h));
+        else if (!wrap)
+          hn = load_hdr(bd, A.blocks_start[(size_t)(m + 1) * nb + b]);
+        if (h.num_stages)
+          run_partition<false, false>(A, bd, h, P, in, out, e, 0u, st_v, st_x, hin_v, hin_x,
+                                      wo_v, wo_x, st_arr, wo_arr);
+        if (last) {
+          wait_vmem();
+          if (hn.num_stages) load_pre(P, bd, hn, in);
+          h = hn;
+          break;
+        }
+        if (hn.num_stages) load_pre(P, bd, hn, in);
+        h = hn;
+      }
+      grid_arrive_wait(A.bar, nb, b, gen, &g_error);
+    }
+  }
+}
+
 thread_local std::string t_err;
 u32 g_slot = 0;
 
@@ -756,5 +818,58 @@ extern "C" int jq_device_error_flag(u32 *out, int clear) {
   }
   if (batched_error_flag(&vb, clear)) return -1;
   *out = v | vb;
+  return 0;
+}
+
+extern "C" size_t jq_cosim_flash_state_bytes(void) { return sizeof(cosim::FlashState); }
+
+extern "C" int jq_cosim_edges(usize num_blocks, usize num_major_stages,
+                              const usize *blocks_start, const u32 *blocks_data,
+                              u32 *sram_data, usize state_size, u32 *states,
+                              const u32 *edge_ops, const u32 *edge_op_start,
+                              u32 num_sched, u32 sched_offset, u32 num_edges,
+                              void *flash_state, const u32 *flash_pins,
+                              const u8 *flash_data, u32 flash_data_size, u32 *trace,
+                              void *stream) {
+  t_err.clear();
+  if (num_edges == 0) return 0;
+  if (num_blocks == 0 || num_blocks > 65535) return fail("cosim: num_blocks out of range");
+  if (!states || state_size == 0 || state_size > 0xFFFFFFFFull / 2)
+    return fail("cosim: state buffer missing or state_size out of range");
+  if (num_sched == 0 || !edge_op_start || !edge_ops) return fail("cosim: empty edge schedule");
+  if (flash_state && !flash_pins) return fail("cosim: flash state without flash pins");
+  if (flash_data_size && !flash_data) return fail("cosim: flash image missing");
+  ScanArgs A{};
+  A.num_blocks = (u32)num_blocks;
+  A.num_major_stages = (u32)num_major_stages;
+  A.blocks_start = blocks_start;
+  A.blocks_data = blocks_data;
+  A.sram_data = sram_data;
+  A.sram_xmask = sram_data;
+  A.num_cycles = 1;
+  A.state_size = state_size;
+  A.states = states;
+  cosim::CosimArgs C{};
+  C.states = states;
+  C.ss = (u32)state_size;
+  C.ops = edge_ops;
+  C.op_start = edge_op_start;
+  C.num_sched = num_sched;
+  C.sched_offset = sched_offset;
+  C.num_edges = num_edges;
+  C.fs = reinterpret_cast<cosim::FlashState *>(flash_state);
+  C.pins = flash_pins;
+  C.flash_data = flash_data;
+  C.flash_size = flash_data_size;
+  C.trace = trace;
+  u32 *bar_base = nullptr;
+  if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(g_barrier)) != hipSuccess)
+    return fail("hipGetSymbolAddress(g_barrier) failed");
+  A.bar = bar_base + kCohSlotWords * (__atomic_fetch_add(&g_slot, 1u, __ATOMIC_RELAXED) % kCohSlots);
+  void *args[] = {&A, &C};
+  hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&cosim_kernel),
+                                            dim3(A.num_blocks), dim3(256), args, 0,
+                                            (hipStream_t)stream);
+  if (e != hipSuccess) return fail(std::string("cosim launch failed: ") + hipGetErrorString(e));
   return 0;
 }

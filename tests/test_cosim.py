@@ -22,7 +22,7 @@ sys.path.insert(0, os.path.join(HERE, "..", "oracle"))
 sys.path.insert(0, HERE)
 
 import cosim_oracle as co  # noqa: E402
-from cosim_design import spi_master_verilog  # noqa: E402
+
 from jacquard_amd import cosim  # noqa: E402
 
 
@@ -165,7 +165,7 @@ def spi_setup(address, image, reset_edges=8):
     """(script, pins words, ops builder, output bits) of the SPI master."""
     from jacquard_amd.netlist import Netlist
     from jacquard_amd.script import Compiled
-    nl = Netlist.from_verilog(text=spi_master_verilog(address))
+    nl = Netlist.from_verilog(text=cosim.spi_master_verilog(address))
     s = Compiled(nl).flatten(1)
     ins, outs = nl.ports()
     inbit = {n: s.input_map[p] for n, p, _ in ins}
