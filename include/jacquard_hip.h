@@ -213,16 +213,24 @@ int jq_lds_read_probe(u32 num_blocks, u32 iters, u32 *sink, void *stream);
  *      layout, csrc/kernel_v1.metal:664-667);
  *   2. the SPI flash's data outputs d_i are written to its data-in bits;
  *   3. one cycle of the scan kernel over the two snapshots;
- *   4. the SPI flash model steps on the edge's outputs;
+ *   4. the SPI flash model steps on the edge's outputs, then the UART
+ *      decoders and the Wishbone bus trace (gpu_io_step,
+ *      csrc/kernel_v1.metal:1103-1231);
  *   5. with trace != NULL, both snapshots are copied to trace[e*2ss ..].
- * state_size = ss (2-state, no timing).  flash_state: the reference's
- * 48-byte FlashState (jq_cosim_flash_state_bytes(), csrc/kernel_v1.metal:
- * 731-749) or NULL for no flash; flash_pins: 11 u32 = data-in bits d0..d3,
- * SPI clock output bit, chip-select output bit, data-out bits d0..d3
- * (0xFFFFFFFF: not connected), has_flash; flash_data: the flash image of
- * flash_data_size bytes.  All pointers are device pointers; the host keeps
- * FlashState.in_reset and the reset bit operations current between calls.
- * Bit positions past the snapshot are ignored.  Asynchronous on `stream`. */
+ * The batch is one persistent cooperative launch (num_blocks workgroups, as
+ * jq_simulate_scan).  state_size = ss (2-state, no timing).  flash_state:
+ * the reference's 48-byte FlashState (jq_cosim_flash_state_bytes(),
+ * csrc/kernel_v1.metal:731-749) or NULL for no flash; flash_pins: 11 u32 =
+ * data-in bits d0..d3, SPI clock output bit, chip-select output bit,
+ * data-out bits d0..d3 (0xFFFFFFFF: not connected), has_flash; flash_data:
+ * the flash image of flash_data_size bytes.  uart_state / uart_params /
+ * uart_channels: the reference's UartDecoderState[4], UartParams and
+ * UartChannel[4] (csrc/kernel_v1.metal:1027-1053), or NULL;
+ * bus_trace / bus_trace_params: its WbTraceChannel (entries follow the
+ * 16-byte header) and WbTraceParams (:1061-1101), or NULL.  All pointers
+ * are device pointers; the host keeps FlashState.in_reset and the reset bit
+ * operations current between calls.  Bit positions past the snapshot are
+ * ignored.  Asynchronous on `stream`. */
 size_t jq_cosim_flash_state_bytes(void);
 int jq_cosim_edges(usize num_blocks, usize num_major_stages,
                    const usize *blocks_start, const u32 *blocks_data,
@@ -230,7 +238,9 @@ int jq_cosim_edges(usize num_blocks, usize num_major_stages,
                    const u32 *edge_ops, const u32 *edge_op_start, u32 num_sched,
                    u32 sched_offset, u32 num_edges, void *flash_state,
                    const u32 *flash_pins, const u8 *flash_data,
-                   u32 flash_data_size, u32 *trace, void *stream);
+                   u32 flash_data_size, void *uart_state, const void *uart_params,
+                   void *uart_channels, void *bus_trace,
+                   const void *bus_trace_params, u32 *trace, void *stream);
 
 /* ---- Learned cost model MLP (no counterpart in the reference) -----------
  * Fully connected FP32 network in -> h1 -> h2 -> 1 with ReLU (in <= 32,

@@ -23,6 +23,31 @@ import numpy as np
 from . import _native as N
 
 FLASH_STATE_BYTES = 48
+# UART and bus-trace layouts (csrc/kernel_v1.metal:1027-1101)
+UART_MAX = 4
+UART_CHANNEL_CAP = 4096
+UART_PARAMS_WORDS = 12
+UART_CHANNEL_WORDS = 4 + UART_CHANNEL_CAP // 4
+WB_ADR_BITS = 30
+WB_DAT_BITS = 32
+BUS_SIGNALS = ("ibus_cyc", "ibus_stb", "ibus_adr", "ibus_rdata", "dbus_cyc", "dbus_stb",
+               "dbus_we", "dbus_adr", "spiflash_ack", "sram_ack", "csr_ack")
+
+
+def bus_trace_params(sig):
+    """WbTraceParams words from {signal: output state bit (lists for the
+    address and data buses)}; missing signals are not connected."""
+    words = []
+    for name in BUS_SIGNALS:
+        width = {"ibus_adr": WB_ADR_BITS, "dbus_adr": WB_ADR_BITS,
+                 "ibus_rdata": WB_DAT_BITS}.get(name)
+        if width is None:
+            words.append(sig.get(name, NC))
+        else:
+            bits = list(sig.get(name, ()))[:width]
+            words += bits + [NC] * (width - len(bits))
+    words.append(1)  # has_trace
+    return np.asarray(words, np.uint32)
 _IN_RESET_OFFSET = 36  # FlashState.in_reset (csrc/kernel_v1.metal:731-749)
 NC = 0xFFFFFFFF
 
@@ -144,7 +169,7 @@ class Cosim:
 
     def __init__(self, script, scheduler, domains, *, reset_pos=None, reset_active=1,
                  reset_edges=0, constants=(), model_driven=(), flash_pins=None,
-                 flash_image=b"", stream=None):
+                 flash_image=b"", uarts=(), bus_trace=None, stream=None):
         import torch
         self.torch = torch
         self.script = script
@@ -185,6 +210,32 @@ class Cosim:
             self.d_image = torch.from_numpy(img.copy()).to(dev)
             self.image_size = len(flash_image)
         self._ops_reset_val = None
+        # UART decoders: [(tx output bit, edges per bit)], at most 4
+        self.uarts = list(uarts)
+        if len(self.uarts) > UART_MAX:
+            raise ValueError(f"at most {UART_MAX} UARTs")
+        if self.uarts:
+            prm = np.zeros(UART_PARAMS_WORDS, np.uint32)
+            prm[0] = self.ss
+            prm[1] = len(self.uarts)
+            for i, (pos, cpb) in enumerate(self.uarts):
+                prm[4 + 2 * i] = pos
+                prm[5 + 2 * i] = cpb
+            us = np.zeros((UART_MAX, 6), np.uint32)
+            us[:, 1] = 1  # last_tx: the line idles high
+            ch = np.zeros((UART_MAX, UART_CHANNEL_WORDS), np.uint32)
+            ch[:, 1] = UART_CHANNEL_CAP
+            self.d_uart_params = torch.from_numpy(prm.view(np.int32)).to(dev)
+            self.d_uart_state = torch.from_numpy(us.view(np.int32)).to(dev)
+            self.d_uart_ch = torch.from_numpy(ch.view(np.int32)).to(dev)
+        # Wishbone bus trace: {signal: state bit or list of bits} and capacity
+        self.bus = bus_trace is not None
+        if self.bus:
+            self.d_bus_params = torch.from_numpy(bus_trace_params(bus_trace).view(np.int32)).to(dev)
+            cap = int(bus_trace.get("capacity", 16384))
+            hdr = np.zeros(4 + 5 * cap, np.uint32)
+            hdr[1] = cap
+            self.d_bus = torch.from_numpy(hdr.view(np.int32)).to(dev)
 
     def _upload_ops(self, reset_val):
         if reset_val == self._ops_reset_val:
@@ -226,6 +277,11 @@ class Cosim:
                 self.d_pins.data_ptr() if self.flash else None,
                 self.d_image.data_ptr() if self.flash else None,
                 self.image_size if self.flash else 0,
+                self.d_uart_state.data_ptr() if self.uarts else None,
+                self.d_uart_params.data_ptr() if self.uarts else None,
+                self.d_uart_ch.data_ptr() if self.uarts else None,
+                self.d_bus.data_ptr() if self.bus else None,
+                self.d_bus_params.data_ptr() if self.bus else None,
                 tr.data_ptr() if tr is not None else None,
                 ctypes.c_void_p(self.stream.cuda_stream)))
             if tr is not None:
@@ -247,6 +303,27 @@ class Cosim:
 
     def snapshots(self):
         return self.states.cpu().numpy().view(np.uint32).copy()
+
+    def uart_bytes(self, i):
+        """Bytes decoded by UART i so far (the last UART_CHANNEL_CAP when
+        the channel wrapped)."""
+        ch = self.d_uart_ch[i].cpu().numpy().view(np.uint32)
+        head = int(ch[0])
+        data = ch[4:].view(np.uint8)
+        if head <= UART_CHANNEL_CAP:
+            return bytes(data[:head])
+        k = head % UART_CHANNEL_CAP
+        return bytes(np.concatenate([data[k:], data[:k]]))
+
+    def uart_states(self):
+        return self.d_uart_state.cpu().numpy().view(np.uint32).copy()
+
+    def bus_entries(self):
+        """(header [write_head, capacity, current_tick, prev_flags],
+        entries uint32[n, 5] = tick, flags, ibus_adr, ibus_rdata, dbus_adr)."""
+        a = self.d_bus.cpu().numpy().view(np.uint32)
+        n = min(int(a[0]), int(a[1]))
+        return a[:4].copy(), a[4:4 + 5 * n].reshape(n, 5).copy()
 
 
 # ---- synthetic co-simulation workload --------------------------------------
@@ -376,5 +453,61 @@ def spi_master_verilog(address):
     L.append(f"  assign flash_d0 = {net(mosi)};")
     for i in range(16):
         L.append(f"  assign rx_o{i} = rx{i};")
+    L.append("endmodule")
+    return "\n".join(L) + "\n"
+
+
+def uart_tx_verilog(data, first=4, cycles_per_bit=2, counter_bits=8):
+    """A gate-level UART transmitter with a known answer: a saturating
+    counter c (one per clock cycle) drives `tx` idle high, then for byte j
+    of `data` a start bit, 8 data bits LSB first and a stop bit, each
+    `cycles_per_bit` cycles long, from cycle `first` on.  Bus-shaped outputs
+    for a trace: cyc (c % 8 in 1..3), stb (c % 8 in 2..3), we (bit 3 of c),
+    ack (c % 8 == 3) and adr0..7 = c.  Ports: clk, rst; tx, cyc, stb, we,
+    ack, adr0..7."""
+    n = 1 << counter_bits
+    frame = 10 * cycles_per_bit
+    if first + frame * len(data) > n - 1:
+        raise ValueError("too many bytes for the counter")
+
+    def tx(c):
+        k = c - first
+        if k < 0 or k >= frame * len(data):
+            return 1
+        j, bit = divmod(k // cycles_per_bit, 10)
+        if bit == 0:
+            return 0
+        if bit == 9:
+            return 1
+        return (data[j] >> (bit - 1)) & 1
+
+    g = _Gates()
+    cbits = [f"c{i}" for i in range(counter_bits)]
+    memo = {}
+
+    def f(pred):
+        return g.func([bool(pred(c)) for c in range(n)], cbits, memo)
+
+    def net(x):
+        if isinstance(x, bool):
+            return "1'b1" if x else "1'b0"
+        return x
+
+    outs = {"tx": f(tx), "cyc": f(lambda c: c % 8 in (1, 2, 3)),
+            "stb": f(lambda c: c % 8 in (2, 3)), "we": f(lambda c: (c >> 3) & 1),
+            "ack": f(lambda c: c % 8 == 3)}
+    for i in range(counter_bits):
+        outs[f"adr{i}"] = f"c{i}"
+    dnext = [g.and2(f(lambda c, i=i: (min(c + 1, n - 1) >> i) & 1), "rst", 0, 1, 0)
+             for i in range(counter_bits)]
+    L = [f"module uart_tx(clk, rst, {', '.join(outs)});", "  input clk;", "  input rst;"]
+    L += [f"  output {o};" for o in outs]
+    L += [f"  wire {w};" for w in cbits + [f"d{i}" for i in range(counter_bits)] + g.wires]
+    L += g.lines
+    for i in range(counter_bits):
+        L.append(f"  assign d{i} = {dnext[i]};")
+        L.append(f"  DFF q{i} (.CLK(clk), .D(d{i}), .Q(c{i}));")
+    for o, x in outs.items():
+        L.append(f"  assign {o} = {net(x)};")
     L.append("endmodule")
     return "\n".join(L) + "\n"

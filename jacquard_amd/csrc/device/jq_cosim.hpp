@@ -35,6 +35,44 @@ static_assert(sizeof(FlashState) == 48, "FlashState layout");
 // four data inputs d0..d3 (0xFFFFFFFF: not connected), of the SPI clock and
 // chip-select outputs and of the four data outputs, then has_flash.
 
+// UART decoders and the Wishbone bus trace (csrc/kernel_v1.metal:1027-1101),
+// the same layouts: up to 4 UARTs, 4096-byte channels, 30 address and 32
+// data bits.
+constexpr u32 kMaxUarts = 4;
+constexpr u32 kUartChannelCap = 4096;
+constexpr u32 kWbAdrBits = 30;
+constexpr u32 kWbDatBits = 32;
+
+struct UartState {
+  u32 state;  // 0 idle, 1 start bit, 2 data bits, 3 stop bit
+  u32 last_tx, start_cycle, bits_received, value, current_cycle;
+};
+struct UartParams {
+  u32 state_size, n_uarts, pad[2];
+  struct {
+    u32 tx_out_pos, cycles_per_bit;
+  } ch[kMaxUarts];
+};
+struct UartChannel {
+  u32 write_head, capacity, pad[2];
+  u8 data[kUartChannelCap];
+};
+struct WbTraceParams {
+  u32 ibus_cyc, ibus_stb, ibus_adr[kWbAdrBits], ibus_rdata[kWbDatBits];
+  u32 dbus_cyc, dbus_stb, dbus_we, dbus_adr[kWbAdrBits];
+  u32 spiflash_ack, sram_ack, csr_ack, has_trace;
+};
+struct WbTraceEntry {
+  u32 tick, flags, ibus_adr, ibus_rdata, dbus_adr;
+};
+struct WbTraceChannel {
+  u32 write_head, capacity, current_tick, prev_flags;  // entries follow
+};
+static_assert(sizeof(UartState) == 24 && sizeof(UartParams) == 48, "UART layouts");
+static_assert(sizeof(UartChannel) == 16 + kUartChannelCap, "UART channel layout");
+static_assert(sizeof(WbTraceParams) == 4 * (2 + 30 + 32 + 3 + 30 + 4), "trace layout");
+static_assert(sizeof(WbTraceEntry) == 20 && sizeof(WbTraceChannel) == 16, "trace layout");
+
 struct CosimArgs {
   u32 *states;
   u32 ss;
@@ -46,6 +84,11 @@ struct CosimArgs {
   const u8 *flash_data;
   u32 flash_size;
   u32 *trace;            // null, or num_edges x 2*ss words
+  UartState *us;         // null: no UART decoding
+  const UartParams *up;
+  UartChannel *uc;
+  WbTraceChannel *wbc;   // null: no bus trace
+  const WbTraceParams *wbp;
 };
 
 // Bit access inside one snapshot of ss words; positions past its end read
@@ -155,6 +198,86 @@ __device__ void flash_step(const CosimArgs &A) {
   *A.fs = f;
 }
 
+// The UART decoders and the bus trace after an edge (gpu_io_step,
+// csrc/kernel_v1.metal:1103-1231): each UART samples its TX output once per
+// edge (a start bit confirmed at half a bit, data bits at bit centres, LSB
+// first, the byte kept when the stop bit reads 1); the trace records the
+// bus signals when a bus cycle is active or any flag changed.
+__device__ void io_step(const CosimArgs &A) {
+  const u32 *out = A.states + A.ss;
+  if (A.us && A.up && A.uc) {
+    const u32 n = A.up->n_uarts < kMaxUarts ? A.up->n_uarts : kMaxUarts;
+    for (u32 ui = 0; ui < n; ++ui) {
+      const u32 cpb = A.up->ch[ui].cycles_per_bit;
+      const u32 tx = bit_at(out, A.ss, A.up->ch[ui].tx_out_pos);
+      UartState u = A.us[ui];
+      const u32 cyc = u.current_cycle;
+      if (u.state == 0) {
+        if (u.last_tx == 1 && tx == 0) {
+          u.state = 1;
+          u.start_cycle = cyc;
+        }
+      } else if (u.state == 1) {
+        if (cyc >= u.start_cycle + cpb / 2) {
+          if (tx == 0) {
+            u.state = 2;
+            u.start_cycle += cpb;
+            u.bits_received = 0;
+            u.value = 0;
+          } else {
+            u.state = 0;
+          }
+        }
+      } else if (u.state == 2) {
+        if (cyc >= u.start_cycle + u.bits_received * cpb + cpb / 2) {
+          u.value |= tx << u.bits_received;
+          if (u.bits_received >= 7) {
+            u.state = 3;
+            u.start_cycle += 8 * cpb;
+          } else {
+            ++u.bits_received;
+          }
+        }
+      } else if (u.state == 3) {
+        if (cyc >= u.start_cycle + cpb / 2) {
+          if (tx == 1) {
+            UartChannel &c = A.uc[ui];
+            const u32 cap = c.capacity - 1u < kUartChannelCap ? c.capacity : kUartChannelCap;
+            c.data[c.write_head % cap] = (u8)(u.value & 0xFFu);
+            c.write_head += 1;
+          }
+          u.state = 0;
+        }
+      }
+      u.last_tx = tx;
+      u.current_cycle = cyc + 1;
+      A.us[ui] = u;
+    }
+  }
+  if (A.wbc && A.wbp && A.wbp->has_trace) {
+    const WbTraceParams &P = *A.wbp;
+    const u32 ic = bit_at(out, A.ss, P.ibus_cyc), is = bit_at(out, A.ss, P.ibus_stb);
+    const u32 dc = bit_at(out, A.ss, P.dbus_cyc), ds = bit_at(out, A.ss, P.dbus_stb);
+    const u32 flags = ic | is << 1 | dc << 2 | ds << 3 | bit_at(out, A.ss, P.dbus_we) << 4 |
+                      bit_at(out, A.ss, P.spiflash_ack) << 5 |
+                      bit_at(out, A.ss, P.sram_ack) << 6 | bit_at(out, A.ss, P.csr_ack) << 7;
+    WbTraceChannel &c = *A.wbc;
+    if ((ic && is) || (dc && ds) || flags != c.prev_flags) {
+      u32 ia = 0, id = 0, da = 0;
+      for (u32 i = 0; i < kWbAdrBits; ++i) ia |= bit_at(out, A.ss, P.ibus_adr[i]) << i;
+      for (u32 i = 0; i < kWbDatBits; ++i) id |= bit_at(out, A.ss, P.ibus_rdata[i]) << i;
+      for (u32 i = 0; i < kWbAdrBits; ++i) da |= bit_at(out, A.ss, P.dbus_adr[i]) << i;
+      if (c.write_head < c.capacity) {
+        WbTraceEntry *e = reinterpret_cast<WbTraceEntry *>(&c + 1) + c.write_head;
+        *e = WbTraceEntry{c.current_tick, flags, ia, id, da};
+        c.write_head += 1;
+      }
+    }
+    c.prev_flags = flags;
+    c.current_tick += 1;
+  }
+}
+
 // dst[i] = src[i] for i < n over one workgroup, src read agent-coherently
 // (dst written so when `coherent`), with kCopyBatch loads in flight per
 // thread before their stores.
@@ -188,7 +311,10 @@ __device__ __forceinline__ void copy_words(u32 *dst, const u32 *src, u32 n, u32 
 __device__ void between_phase(const CosimArgs &A, u32 t, bool step, u32 prep_edge,
                               u32 *trace) {
   const bool flash = A.fs != nullptr && A.pins[10] != 0;
-  if (step && flash && t == 0) flash_step(A);
+  if (step && t == 0) {
+    if (flash) flash_step(A);
+    io_step(A);
+  }
   if (trace) copy_words(trace, A.states, 2 * A.ss, t, false);
   if (prep_edge == ~0u) return;
   __syncthreads();

@@ -829,10 +829,15 @@ extern "C" int jq_cosim_edges(usize num_blocks, usize num_major_stages,
                               const u32 *edge_ops, const u32 *edge_op_start,
                               u32 num_sched, u32 sched_offset, u32 num_edges,
                               void *flash_state, const u32 *flash_pins,
-                              const u8 *flash_data, u32 flash_data_size, u32 *trace,
-                              void *stream) {
+                              const u8 *flash_data, u32 flash_data_size,
+                              void *uart_state, const void *uart_params,
+                              void *uart_channels, void *bus_trace,
+                              const void *bus_trace_params, u32 *trace, void *stream) {
   t_err.clear();
   if (num_edges == 0) return 0;
+  if (uart_state && (!uart_params || !uart_channels))
+    return fail("cosim: UART state without its parameters or channels");
+  if (bus_trace && !bus_trace_params) return fail("cosim: bus trace without its parameters");
   if (num_blocks == 0 || num_blocks > 65535) return fail("cosim: num_blocks out of range");
   if (!states || state_size == 0 || state_size > 0xFFFFFFFFull / 2)
     return fail("cosim: state buffer missing or state_size out of range");
@@ -862,6 +867,11 @@ extern "C" int jq_cosim_edges(usize num_blocks, usize num_major_stages,
   C.flash_data = flash_data;
   C.flash_size = flash_data_size;
   C.trace = trace;
+  C.us = reinterpret_cast<cosim::UartState *>(uart_state);
+  C.up = reinterpret_cast<const cosim::UartParams *>(uart_params);
+  C.uc = reinterpret_cast<cosim::UartChannel *>(uart_channels);
+  C.wbc = reinterpret_cast<cosim::WbTraceChannel *>(bus_trace);
+  C.wbp = reinterpret_cast<const cosim::WbTraceParams *>(bus_trace_params);
   u32 *bar_base = nullptr;
   if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(g_barrier)) != hipSuccess)
     return fail("hipGetSymbolAddress(g_barrier) failed");

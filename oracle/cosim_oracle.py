@@ -12,8 +12,10 @@ Follows:
     csrc/kernel_v1.metal:929-1025;
   * state preparation and flash data-in, csrc/kernel_v1.metal:678-710 and
     896-918;
+  * the UART decoders and the Wishbone bus trace, csrc/kernel_v1.metal:
+    1103-1231;
   * the edge order of src/sim/cosim_metal.rs:574-650 (prepare, data-in,
-    simulate, flash step, snapshot copy).
+    simulate, flash step, UART and bus trace, snapshot copy).
 
 `Flash.step_model` is checked against the reference's own model compiled
 from csrc/spiflash_model.cc (oracle/_ref/libspiflash_ref.so) where that
@@ -144,6 +146,88 @@ class Flash:
         return r
 
 
+class Uart:
+    """One UART TX decoder, sampled once per edge (csrc/kernel_v1.metal:
+    1120-1178), with its byte channel."""
+
+    def __init__(self, tx_pos, cycles_per_bit):
+        self.tx_pos = tx_pos
+        self.cpb = cycles_per_bit
+        self.state = 0
+        self.last_tx = 1
+        self.start_cycle = 0
+        self.bits_received = 0
+        self.value = 0
+        self.current_cycle = 0
+        self.data = []
+
+    def step(self, tx):
+        cyc, cpb = self.current_cycle, self.cpb
+        if self.state == 0:
+            if self.last_tx == 1 and tx == 0:
+                self.state, self.start_cycle = 1, cyc
+        elif self.state == 1:
+            if cyc >= self.start_cycle + cpb // 2:
+                if tx == 0:
+                    self.state = 2
+                    self.start_cycle += cpb
+                    self.bits_received = 0
+                    self.value = 0
+                else:
+                    self.state = 0
+        elif self.state == 2:
+            if cyc >= self.start_cycle + self.bits_received * cpb + cpb // 2:
+                self.value |= tx << self.bits_received
+                if self.bits_received >= 7:
+                    self.state = 3
+                    self.start_cycle += 8 * cpb
+                else:
+                    self.bits_received += 1
+        elif self.state == 3:
+            if cyc >= self.start_cycle + cpb // 2:
+                if tx == 1:
+                    self.data.append(self.value & 0xFF)
+                self.state = 0
+        self.last_tx = tx
+        self.current_cycle = cyc + 1
+
+    def record(self):
+        return [self.state, self.last_tx, self.start_cycle, self.bits_received,
+                self.value, self.current_cycle]
+
+
+class BusTrace:
+    """The Wishbone bus trace (csrc/kernel_v1.metal:1181-1229)."""
+
+    def __init__(self, sig, capacity):
+        self.sig = sig
+        self.capacity = capacity
+        self.entries = []
+        self.write_head = 0
+        self.current_tick = 0
+        self.prev_flags = 0
+
+    def step(self, out):
+        def b(name):
+            return _bit(out, self.sig[name]) if name in self.sig else 0
+
+        def bus(name, width):
+            bits = list(self.sig.get(name, ()))[:width]
+            return sum(_bit(out, p) << i for i, p in enumerate(bits))
+
+        names = ("ibus_cyc", "ibus_stb", "dbus_cyc", "dbus_stb", "dbus_we",
+                 "spiflash_ack", "sram_ack", "csr_ack")
+        flags = sum(b(n) << i for i, n in enumerate(names))
+        active = (flags & 3) == 3 or (flags & 12) == 12
+        if active or flags != self.prev_flags:
+            if self.write_head < self.capacity:
+                self.entries.append((self.current_tick, flags, bus("ibus_adr", 30),
+                                     bus("ibus_rdata", 32), bus("dbus_adr", 30)))
+                self.write_head += 1
+        self.prev_flags = flags
+        self.current_tick += 1
+
+
 def _bit(words, pos):
     return int((int(words[pos >> 5]) >> (pos & 31)) & 1) if (pos >> 5) < len(words) else 0
 
@@ -159,7 +243,7 @@ def _put(words, pos, v):
 
 
 def run_edges(script, states, ops, op_start, sched_offset, num_edges, flash=None,
-              pins=None):
+              pins=None, uarts=(), bus=None):
     """The edge loop over a two-snapshot buffer (uint32[2*ss], updated in
     place).  ops: uint32[n, 2] (state bit, value); op_start: CSR over the
     schedule edges.  pins: 11 ints as jq_cosim_edges takes them.  Returns
@@ -183,6 +267,11 @@ def run_edges(script, states, ops, op_start, sched_offset, num_edges, flash=None
             out = states[ss:]
             d_out = sum(_bit(out, int(pins[6 + i])) << i for i in range(4))
             flash.cosim_step(_bit(out, int(pins[4])), _bit(out, int(pins[5])), d_out)
+        out = states[ss:]
+        for u in uarts:
+            u.step(_bit(out, u.tx_pos))
+        if bus is not None:
+            bus.step(out)
         trace[e] = states
     return trace
 

@@ -231,3 +231,78 @@ def test_flash_held_in_reset_reads_nothing():
                  np.asarray(start, np.uint32), 0, 300, fl, pins.words())
     rx = sum(co._bit(st[ss:], outbit[f"rx_o{i}"]) << i for i in range(16))
     assert rx == 0xFFFF and fl.byte_count == 0
+
+
+def test_uart_decoder_closed_forms():
+    # a TX waveform sampled once per edge, 8 edges per bit: bytes framed
+    # with start and stop bits decode LSB first; a frame whose stop bit
+    # reads 0 is dropped; a start bit that is high again at half a bit is a
+    # glitch and returns to idle
+    cpb = 8
+
+    def frame(v, stop=1):
+        bits = [0] + [(v >> i) & 1 for i in range(8)] + [stop]
+        return [b for b in bits for _ in range(cpb)]
+
+    wave = [1] * 5 + frame(0x41) + [1] * 3 + frame(0xC3) + frame(0x55, stop=0) + [1] * 20
+    wave += [0] * 3 + [1] * 30 + frame(0x7E) + [1] * 10
+    u = co.Uart(0, cpb)
+    for tx in wave:
+        u.step(tx)
+    assert bytes(u.data) == bytes([0x41, 0xC3, 0x7E])
+    assert u.state == 0 and u.current_cycle == len(wave)
+
+
+def test_bus_trace_closed_forms():
+    # flags of (ibus_cyc, ibus_stb, dbus_cyc, dbus_stb, ...) at bits 0..7 of
+    # word 0; the address at word 1: an entry on every active tick and on
+    # every flag change, up to the capacity
+    sig = {"ibus_cyc": 0, "ibus_stb": 1, "dbus_cyc": 2, "dbus_stb": 3, "dbus_we": 4,
+           "ibus_adr": list(range(32, 62))}
+    seq = [0b0000, 0b0001, 0b0011, 0b0011, 0b0000, 0b0000, 0b0100, 0b1100, 0b11100,
+           0b0000]
+    bt = co.BusTrace(sig, capacity=6)
+    for k, fl in enumerate(seq):
+        out = np.asarray([fl, 100 + k], np.uint32)
+        bt.step(out)
+    # ticks: 1 changed, 2 active+changed, 3 active, 4 changed, 6 changed,
+    # 7 active (dbus), 8 active, 9 changed: capacity 6 keeps the first six
+    assert [e[0] for e in bt.entries] == [1, 2, 3, 4, 6, 7]
+    assert [e[2] for e in bt.entries] == [101, 102, 103, 104, 106, 107]
+    assert bt.write_head == 6 and bt.current_tick == len(seq) and bt.prev_flags == 0
+
+
+def uart_setup(data):
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    nl = Netlist.from_verilog(text=cosim.uart_tx_verilog(data))
+    s = Compiled(nl).flatten(1)
+    ins, outs = nl.ports()
+    inbit = {n: s.input_map[p] for n, p, _ in ins}
+    outbit = {n: s.output_map[lit] for n, lit in outs if lit > 1}
+    sig = {"ibus_cyc": outbit["cyc"], "ibus_stb": outbit["stb"], "dbus_we": outbit["we"],
+           "csr_ack": outbit["ack"], "ibus_adr": [outbit[f"adr{i}"] for i in range(8)],
+           "dbus_adr": [outbit[f"adr{7 - i}"] for i in range(8)]}
+    return s, inbit, outbit, sig
+
+
+def test_uart_tx_decodes_through_the_oracle_loop():
+    data = bytes([0x48, 0x00, 0xFF, 0xA5, 0x5A, 0x7F])
+    s, inbit, outbit, sig = uart_setup(data)
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
+    doms = [d for _, d in cosim.clock_domains_of(s)]
+    ops, start = [], [0]
+    for k in range(len(sched)):
+        ops += sched.edge_ops(k, doms, inbit["rst"], 0)
+        start.append(len(ops))
+    st = np.zeros(2 * s.effective_state_size(), np.uint32)
+    u = co.Uart(outbit["tx"], 4)  # 2 clock cycles per bit = 4 edges
+    bus = co.BusTrace(sig, 1000)
+    co.run_edges(s, st, np.asarray(ops, np.uint32).reshape(-1, 2),
+                 np.asarray(start, np.uint32), 0, 520, uarts=[u], bus=bus)
+    assert bytes(u.data) == data
+    # every entry holds the counter on the ibus address and its bit reverse
+    # on the dbus address
+    assert len(bus.entries) > 0 and all(e[2] == sum(((e[4] >> (7 - i)) & 1) << i
+                                                    for i in range(8))
+                                        for e in bus.entries)

@@ -75,3 +75,37 @@ def test_cosim_random_netlist_two_clock_domains():
     c2 = cosim.Cosim(s, sched, doms, constants=consts, model_driven=driven)
     assert c2.run(edges, batch=edges) is None
     assert np.array_equal(c2.snapshots(), st)
+
+
+def test_uart_and_bus_trace_match_oracle():
+    from test_cosim import uart_setup
+    data = bytes(range(0x41, 0x41 + 12))
+    s, inbit, outbit, sig = uart_setup(data)
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
+    doms = [d for _, d in cosim.clock_domains_of(s)]
+    uarts = [(outbit["tx"], 4), (outbit["we"], 6)]
+    edges, cap = 600, 100
+    print(f"cosim uart + bus trace: {edges} edges, trace capacity {cap}", flush=True)
+    c = cosim.Cosim(s, sched, doms, reset_pos=inbit["rst"], reset_active=1, reset_edges=4,
+                    uarts=uarts, bus_trace=dict(sig, capacity=cap))
+    got = c.run(edges, batch=128, trace=True)
+    ss = s.effective_state_size()
+    st = np.zeros(2 * ss, np.uint32)
+    us = [co.Uart(p, cpb) for p, cpb in uarts]
+    bus = co.BusTrace(sig, cap)
+    exp = []
+    for in_reset, n, off in ((1, 4, 0), (0, edges - 4, 4)):
+        ops, start = [], [0]
+        for k in range(len(sched)):
+            ops += sched.edge_ops(k, doms, inbit["rst"], in_reset)
+            start.append(len(ops))
+        exp.append(co.run_edges(s, st, np.asarray(ops, np.uint32).reshape(-1, 2),
+                                np.asarray(start, np.uint32), off % len(sched), n,
+                                uarts=us, bus=bus))
+    assert np.array_equal(got, np.concatenate(exp))
+    assert c.uart_bytes(0) == bytes(us[0].data) == data
+    assert c.uart_bytes(1) == bytes(us[1].data)
+    assert c.uart_states()[:2].tolist() == [u.record() for u in us]
+    hdr, entries = c.bus_entries()
+    assert hdr.tolist() == [bus.write_head, cap, bus.current_tick, bus.prev_flags]
+    assert entries.tolist() == [list(e) for e in bus.entries]
