@@ -511,3 +511,71 @@ def uart_tx_verilog(data, first=4, cycles_per_bit=2, counter_bits=8):
         L.append(f"  assign {o} = {net(x)};")
     L.append("endmodule")
     return "\n".join(L) + "\n"
+
+
+# ---- pin mapping --------------------------------------------------------------
+# Resolution of the co-simulation's GPIO indices and named ports to state bits
+# (build_gpio_mapping, src/sim/cosim_metal.rs:955-1212).
+
+def parse_gpio_index(pin_name, prefix):
+    """N from "prefix[N]" or "prefix:N" anywhere in the name, else None
+    (src/sim/cosim_metal.rs:1214-1233)."""
+    k = pin_name.find(prefix + "[")
+    if k >= 0:
+        rest = pin_name[k + len(prefix) + 1:]
+        end = rest.find("]")
+        if end >= 0:
+            try:
+                return int(rest[:end])
+            except ValueError:
+                return None
+    k = pin_name.find(prefix + ":")
+    if k >= 0:
+        rest = pin_name[k + len(prefix) + 1:]
+        digits = ""
+        for ch in rest:
+            if not ch.isdigit():
+                break
+            digits += ch
+        return int(digits) if digits else None
+    return None
+
+
+@dataclass
+class GpioMapping:
+    input_bits: dict            # gpio index -> input state bit
+    output_bits: dict           # gpio index -> output state bit
+    clock_domains: list         # [(clock port name, gpio index or None, ClockDomain)]
+    named_input_bits: dict      # input port bit name -> state bit
+
+
+def gpio_mapping(script, port_mapping=None):
+    """GPIO indices of the netlist's ports: `port_mapping` = {"inputs":
+    {"N": port bit name}, "outputs": {...}} first, else names of the form
+    gpio_in[N] / gpio_out[N] (or gpio_in:N).  Outputs driven by a constant
+    are skipped; an output whose literal is not in the output map falls back
+    to its complement's position, as the reference does.  One clock domain
+    per clock port, in port order, with its posedge flag."""
+    pm = port_mapping or {}
+    in_name = {v: int(k) for k, v in pm.get("inputs", {}).items() if str(k).isdigit()}
+    out_name = {v: int(k) for k, v in pm.get("outputs", {}).items() if str(k).isdigit()}
+    ins, outs = script.netlist.ports()
+    input_bits, named, domains = {}, {}, []
+    for name, pin, flag in ins:
+        pos = script.input_map.get(pin)
+        g = in_name.get(name, parse_gpio_index(name, "gpio_in"))
+        if pos is not None:
+            named[name] = pos
+            if g is not None:
+                input_bits[g] = pos
+        if flag and flag in script.input_map:
+            domains.append((name, g, ClockDomain(pos, [script.input_map[flag]], [])))
+    output_bits = {}
+    for name, lit in outs:
+        g = out_name.get(name, parse_gpio_index(name, "gpio_out"))
+        if g is None or lit <= 1:
+            continue
+        pos = script.output_map.get(lit, script.output_map.get(lit ^ 1))
+        if pos is not None:
+            output_bits[g] = pos
+    return GpioMapping(input_bits, output_bits, domains, named)

@@ -306,3 +306,52 @@ def test_uart_tx_decodes_through_the_oracle_loop():
     assert len(bus.entries) > 0 and all(e[2] == sum(((e[4] >> (7 - i)) & 1) << i
                                                     for i in range(8))
                                         for e in bus.entries)
+
+
+GPIO_DESIGN = """module soc(clk, gpio_in, gpio_out, led);
+  input clk;
+  input [3:0] gpio_in;
+  output [3:0] gpio_out;
+  output led;
+  wire q0, q1, n1;
+  DFF r0 (.CLK(clk), .D(gpio_in[0]), .Q(q0));
+  DFF r1 (.CLK(clk), .D(n1), .Q(q1));
+  AND2_00_0 g1 (.A(gpio_in[1]), .B(gpio_in[2]), .Y(n1));
+  INV g2 (.A(q1), .Y(gpio_out[1]));
+  assign gpio_out[0] = q0;
+  assign gpio_out[2] = 1'b1;
+  assign gpio_out[3] = gpio_in[3];
+  assign led = q1;
+endmodule
+"""
+
+
+def test_gpio_index_parsing():
+    assert cosim.parse_gpio_index("gpio_in[38]", "gpio_in") == 38
+    assert cosim.parse_gpio_index("top.gpio_in:7_pad", "gpio_in") == 7
+    assert cosim.parse_gpio_index("gpio_out[2]", "gpio_in") is None
+    assert cosim.parse_gpio_index("gpio_in[x]", "gpio_in") is None
+
+
+def test_gpio_mapping_names_ports_and_overrides():
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    nl = Netlist.from_verilog(text=GPIO_DESIGN)
+    s = Compiled(nl).flatten(1)
+    ins, outs = nl.ports()
+    pin = {n: p for n, p, _ in ins}
+    lit = dict(outs)
+    m = cosim.gpio_mapping(s)
+    assert m.input_bits == {i: s.input_map[pin[f"gpio_in[{i}]"]] for i in range(4)}
+    # gpio_out[2] is a constant: skipped; the others resolve through the
+    # output map (gpio_out[3] passes gpio_in[3] through)
+    assert sorted(m.output_bits) == [0, 1, 3]
+    for i in (0, 1, 3):
+        assert m.output_bits[i] == s.output_map[lit[f"gpio_out[{i}]"]]
+    assert [d[0] for d in m.clock_domains] == ["clk"]
+    assert m.named_input_bits["clk"] == s.input_map[pin["clk"]]
+    # an explicit port mapping names led as GPIO 9 and clk as input 5
+    m2 = cosim.gpio_mapping(s, {"inputs": {"5": "clk"}, "outputs": {"9": "led"}})
+    assert m2.input_bits[5] == s.input_map[pin["clk"]]
+    assert m2.clock_domains[0][1] == 5
+    assert m2.output_bits[9] == s.output_map[lit["led"]]
