@@ -355,3 +355,40 @@ def test_gpio_mapping_names_ports_and_overrides():
     assert m2.input_bits[5] == s.input_map[pin["clk"]]
     assert m2.clock_domains[0][1] == 5
     assert m2.output_bits[9] == s.output_map[lit["led"]]
+
+
+def test_cosim_entry_point_rejects_bad_arguments_before_any_device_work():
+    # argument checks return an error (and a message) before any HIP call,
+    # so they run without a GPU
+    import ctypes
+    from jacquard_amd import _native as N
+    L = N.hip()
+    dummy = ctypes.c_void_p(16)
+
+    def call(num_blocks=1, states=dummy, ss=4, ops=dummy, start=dummy, nsched=2,
+             edges=3, fs=None, pins=None, img=None, img_size=0, us=None, up=None, uc=None,
+             bus=None, busp=None):
+        return L.jq_cosim_edges(num_blocks, 1, dummy, dummy, dummy, ss, states, ops, start,
+                                nsched, 0, edges, fs, pins, img, img_size, us, up, uc, bus,
+                                busp, None, None)
+
+    assert call(edges=0) == 0
+    cases = [(dict(num_blocks=0), "num_blocks"), (dict(states=None), "state"),
+             (dict(ss=0), "state"), (dict(nsched=0), "schedule"),
+             (dict(start=None), "schedule"), (dict(fs=dummy), "flash pins"),
+             (dict(img_size=16), "image"), (dict(us=dummy), "UART"),
+             (dict(us=dummy, up=dummy), "UART"), (dict(bus=dummy), "bus trace")]
+    for kw, word in cases:
+        assert call(**kw) != 0, kw
+        assert word in L.jq_last_error().decode(), (kw, L.jq_last_error())
+    assert L.jq_cosim_flash_state_bytes() == 48
+
+
+def test_cosim_runs_the_two_state_script_only():
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    nl = Netlist.from_verilog(text=GPIO_DESIGN)
+    s = Compiled(nl).flatten(1, xprop=True)
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
+    with pytest.raises(ValueError):
+        cosim.Cosim(s, sched, [])
