@@ -84,6 +84,7 @@ struct CosimArgs {
   const u8 *flash_data;
   u32 flash_size;
   u32 *trace;            // null, or num_edges x 2*ss words
+  u32 *fsbuf;            // 2 x 12 words: flash states after even / odd steps
   UartState *us;         // null: no UART decoding
   const UartParams *up;
   UartChannel *uc;
@@ -177,8 +178,7 @@ __device__ void flash_eval(FlashState &f, u32 clk, u32 csn, u32 d_out, u32 &mode
 // system clock edge).  The reference evaluates twice, with the previous and
 // then the current data; the second evaluation sees no clock or chip-select
 // edge and changes nothing, so one evaluation is made here.
-__device__ void flash_step(const CosimArgs &A) {
-  FlashState f = *A.fs;
+__device__ FlashState flash_step(const CosimArgs &A, FlashState f) {
   const u32 *out = A.states + A.ss;
   const u32 clk = bit_at(out, A.ss, A.pins[4]);
   const u32 csn = bit_at(out, A.ss, A.pins[5]);
@@ -195,7 +195,20 @@ __device__ void flash_step(const CosimArgs &A) {
   }
   f.prev_csn = csn;
   f.prev_d_out = (u8)d_out;
-  *A.fs = f;
+  return f;
+}
+
+// FlashState words moved agent-coherently (the double buffer is read by
+// workgroups on other XCDs).
+__device__ __forceinline__ FlashState load_fs(const u32 *p) {
+  FlashState f;
+  u32 *w = reinterpret_cast<u32 *>(&f);
+  for (u32 i = 0; i < sizeof(FlashState) / 4; ++i) w[i] = ld_coherent(p + i);
+  return f;
+}
+__device__ __forceinline__ void store_fs(u32 *p, const FlashState &f) {
+  const u32 *w = reinterpret_cast<const u32 *>(&f);
+  for (u32 i = 0; i < sizeof(FlashState) / 4; ++i) st_coherent(p + i, w[i]);
 }
 
 // The UART decoders and the bus trace after an edge (gpu_io_step,
@@ -304,32 +317,69 @@ __device__ __forceinline__ void copy_words(u32 *dst, const u32 *src, u32 n, u32 
   }
 }
 
-// Between cycle e-1 and cycle e, in one workgroup (t = thread index):
-// the flash step on the outputs of edge e-1 and its trace copy (when
-// flash_step), then the preparation of edge e (prep_edge != ~0u).  The
-// caller waits for this workgroup's stores before the grid barrier.
-__device__ void between_phase(const CosimArgs &A, u32 t, bool step, u32 prep_edge,
-                              u32 *trace) {
+// Between cycle e-1 and cycle e (e = 0: before the first cycle; e =
+// num_edges: after the last), in every workgroup b of nb.  The snapshot
+// words are split in nb ranges; workgroup b, for its range only:
+//   * copies both snapshots of edge e-1 to the trace;
+//   * copies the output snapshot over the input snapshot and applies the
+//     bit operations of edge e that fall in the range, in order, then the
+//     flash's data-in bits that fall in it.
+// The flash model is a pure function of its previous state and the edge's
+// outputs, so every workgroup that owns a data-in word steps it itself from
+// the state of the previous step; workgroup 0 keeps the states, alternately
+// in fsbuf[0..11] and fsbuf[12..23] (state k in buffer k & 1, state 0 = the
+// caller's), so no workgroup reads a buffer that is being written.
+// Workgroup 0 also runs the UART decoders and the bus trace.  The caller
+// waits for the stores and meets the grid barrier afterwards.
+__device__ void between_phase(const CosimArgs &A, u32 t, u32 b, u32 nb, u32 e) {
   const bool flash = A.fs != nullptr && A.pins[10] != 0;
-  if (step && t == 0) {
-    if (flash) flash_step(A);
-    io_step(A);
+  const u32 chunk = (A.ss + nb - 1) / nb;
+  const u32 lo = b * chunk < A.ss ? b * chunk : A.ss;
+  const u32 hi = lo + chunk < A.ss ? lo + chunk : A.ss;
+  const bool last = e == A.num_edges;
+  auto owns = [&](u32 pos) { return (pos >> 5) >= lo && (pos >> 5) < hi; };
+  bool need_fs = false;
+  u32 din = 0;  // the flash's d_i for this edge's input (thread 0)
+  if (flash)
+    for (u32 i = 0; i < 4; ++i) need_fs |= owns(A.pins[i]);
+  if (t == 0) {
+    if (b == 0 && e > 0) io_step(A);
+    if (flash && (b == 0 || need_fs)) {
+      FlashState f;
+      if (e == 0) {
+        f = *A.fs;
+        if (b == 0) store_fs(A.fsbuf, f);
+      } else {
+        f = flash_step(A, load_fs(A.fsbuf + 12 * ((e - 1) & 1)));
+        if (b == 0) {
+          store_fs(A.fsbuf + 12 * (e & 1), f);
+          if (last) *A.fs = f;
+        }
+      }
+      din = f.d_i;
+    }
   }
-  if (trace) copy_words(trace, A.states, 2 * A.ss, t, false);
-  if (prep_edge == ~0u) return;
+  if (e > 0 && A.trace) {
+    u32 *tr = A.trace + (size_t)(e - 1) * 2 * A.ss;
+    copy_words(tr + lo, A.states + lo, hi - lo, t, false);
+    copy_words(tr + A.ss + lo, A.states + A.ss + lo, hi - lo, t, false);
+  }
+  if (last) return;
   __syncthreads();
   // state preparation (csrc/kernel_v1.metal:678-710): the output snapshot
   // becomes the next input, then the edge's bit operations in order
-  copy_words(A.states, A.states + A.ss, A.ss, t, true);
+  copy_words(A.states + lo, A.states + A.ss + lo, hi - lo, t, true);
   wait_vmem();
   __syncthreads();
   if (t == 0) {
-    const u32 b = A.op_start[prep_edge], e = A.op_start[prep_edge + 1];
-    for (u32 k = b; k < e; ++k) put_bit(A.states, A.ss, A.ops[2 * k], A.ops[2 * k + 1]);
+    const u32 pe = (u32)(((usize)A.sched_offset + e) % A.num_sched);
+    const u32 ob = A.op_start[pe], oe = A.op_start[pe + 1];
+    for (u32 k = ob; k < oe; ++k)
+      if (owns(A.ops[2 * k])) put_bit(A.states, A.ss, A.ops[2 * k], A.ops[2 * k + 1]);
     // flash data-in (csrc/kernel_v1.metal:896-918): the model's d_i
-    if (flash) {
-      const u32 d_i = A.fs->d_i;
-      for (u32 i = 0; i < 4; ++i) put_bit(A.states, A.ss, A.pins[i], (d_i >> i) & 1u);
+    if (flash && need_fs) {
+      for (u32 i = 0; i < 4; ++i)
+        if (owns(A.pins[i])) put_bit(A.states, A.ss, A.pins[i], (din >> i) & 1u);
     }
   }
 }

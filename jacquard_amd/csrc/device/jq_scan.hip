@@ -587,8 +587,9 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
   }
 }
 // Persistent co-simulation: C.num_edges edges of one cycle each over the
-// two snapshots of C.states, workgroup 0 doing the between-edges work of
-// jq_cosim.hpp ahead of every cycle (and once after the last).  A cycle
+// two snapshots of C.states, every workgroup doing its share of the
+// between-edges work of jq_cosim.hpp ahead of every cycle (and once after
+// the last).  A cycle
 // walks the partitions as scan_kernel does, except that the first
 // partition of a cycle is loaded after the barrier that follows the
 // preparation, which may change any input word.
@@ -610,12 +611,7 @@ __global__ __launch_bounds__(256) void cosim_kernel(ScanArgs A, cosim::CosimArgs
   const u32 *in = A.states;
   u32 *out = A.states + A.state_size;
   for (u32 e = 0; e <= C.num_edges; ++e) {
-    if (b == 0) {
-      u32 *tr = (e > 0 && C.trace) ? C.trace + (size_t)(e - 1) * 2 * C.ss : nullptr;
-      const u32 prep =
-          e < C.num_edges ? (u32)(((usize)C.sched_offset + e) % C.num_sched) : ~0u;
-      cosim::between_phase(C, (u32)t, e > 0, prep, tr);
-    }
+    cosim::between_phase(C, (u32)t, b, nb, e);
     if (e == C.num_edges) break;
     grid_sync_coherent(A.bar, nb, b, gen, &g_error);
     PartHdr h = M ? load_hdr(bd, A.blocks_start[b]) : no_hdr();
@@ -876,6 +872,9 @@ extern "C" int jq_cosim_edges(usize num_blocks, usize num_major_stages,
   if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(g_barrier)) != hipSuccess)
     return fail("hipGetSymbolAddress(g_barrier) failed");
   A.bar = bar_base + kCohSlotWords * (__atomic_fetch_add(&g_slot, 1u, __ATOMIC_RELAXED) % kCohSlots);
+  // the flash state's double buffer: words 64..87 of the launch's barrier
+  // slot, which the barrier does not use (generation at 32, counters from 256)
+  C.fsbuf = A.bar + 64;
   void *args[] = {&A, &C};
   hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&cosim_kernel),
                                             dim3(A.num_blocks), dim3(256), args, 0,

@@ -161,12 +161,13 @@ def test_edge_ops_order_and_values():
     assert s.edge_ops(2, doms) == [(3, 1), (40, 1), (41, 0), (50, 0)]
 
 
-def spi_setup(address, image, reset_edges=8):
-    """(script, pins words, ops builder, output bits) of the SPI master."""
+def spi_setup(address, image, reset_edges=8, blocks=1):
+    """(script, flash pins, clock domains, input bits, output bits) of the
+    SPI master, flattened for `blocks` workgroups."""
     from jacquard_amd.netlist import Netlist
     from jacquard_amd.script import Compiled
     nl = Netlist.from_verilog(text=cosim.spi_master_verilog(address))
-    s = Compiled(nl).flatten(1)
+    s = Compiled(nl).flatten(blocks)
     ins, outs = nl.ports()
     inbit = {n: s.input_map[p] for n, p, _ in ins}
     outbit = {n: s.output_map[lit] for n, lit in outs if lit > 1}
@@ -272,11 +273,11 @@ def test_bus_trace_closed_forms():
     assert bt.write_head == 6 and bt.current_tick == len(seq) and bt.prev_flags == 0
 
 
-def uart_setup(data):
+def uart_setup(data, blocks=1):
     from jacquard_amd.netlist import Netlist
     from jacquard_amd.script import Compiled
     nl = Netlist.from_verilog(text=cosim.uart_tx_verilog(data))
-    s = Compiled(nl).flatten(1)
+    s = Compiled(nl).flatten(blocks)
     ins, outs = nl.ports()
     inbit = {n: s.input_map[p] for n, p, _ in ins}
     outbit = {n: s.output_map[lit] for n, lit in outs if lit > 1}

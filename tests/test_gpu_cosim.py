@@ -19,11 +19,19 @@ from test_cosim import oracle_edges, spi_setup  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("address,batch", [(0x000123, 64), (0x000FFF, 1000)])
-def test_spi_master_cosim_matches_oracle(address, batch):
+@pytest.mark.parametrize("address,batch,blocks", [(0x000123, 64, 1), (0x000FFF, 1000, 1),
+                                                  (0x000321, 100, 4)])
+def test_spi_master_cosim_matches_oracle(address, batch, blocks):
+    # with 4 workgroups the snapshot words (9) are prepared in ranges of 3,
+    # and the data-in word's owner steps the flash model itself
     rng = np.random.default_rng(address)
     image = bytes(rng.integers(0, 256, 4096, dtype=np.uint8))
-    s, pins, doms, inbit, outbit = spi_setup(address, image)
+    s, pins, doms, inbit, outbit = spi_setup(address, image, blocks=blocks)
+    assert s.num_blocks == blocks
+    if blocks > 1:
+        # d_i bit 0 also drives a bit of the last-but-one state word, which
+        # the last workgroup's range holds: that workgroup steps the model
+        pins.d_in = (32 * (s.effective_state_size() - 2) + 5,) + tuple(pins.d_in[1:])
     sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
     edges, reset = 300, 8
     print(f"cosim spi master: address {address:#x}, {edges} edges in batches of {batch}",
@@ -37,10 +45,11 @@ def test_spi_master_cosim_matches_oracle(address, batch):
     assert bad.size == 0, f"first differing edge {bad[:1]}"
     assert np.array_equal(c.snapshots(), st)
     assert bytes(c.flash_state()) == fl.to_record().tobytes()
-    ss = s.effective_state_size()
-    rx = sum(co._bit(st[ss:], outbit[f"rx_o{i}"]) << i for i in range(16))
-    lo = image[address + 1] if address + 1 < len(image) else 0xFF
-    assert rx == (image[address] << 8 | lo)
+    if blocks == 1:
+        ss = s.effective_state_size()
+        rx = sum(co._bit(st[ss:], outbit[f"rx_o{i}"]) << i for i in range(16))
+        lo = image[address + 1] if address + 1 < len(image) else 0xFF
+        assert rx == (image[address] << 8 | lo)
 
 
 def test_cosim_random_netlist_two_clock_domains():
@@ -77,10 +86,11 @@ def test_cosim_random_netlist_two_clock_domains():
     assert np.array_equal(c2.snapshots(), st)
 
 
-def test_uart_and_bus_trace_match_oracle():
+@pytest.mark.parametrize("blocks", [1, 3])
+def test_uart_and_bus_trace_match_oracle(blocks):
     from test_cosim import uart_setup
     data = bytes(range(0x41, 0x41 + 12))
-    s, inbit, outbit, sig = uart_setup(data)
+    s, inbit, outbit, sig = uart_setup(data, blocks=blocks)
     sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
     doms = [d for _, d in cosim.clock_domains_of(s)]
     uarts = [(outbit["tx"], 4), (outbit["we"], 6)]
