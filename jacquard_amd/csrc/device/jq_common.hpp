@@ -56,7 +56,8 @@ __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" 
 // A slot is kCohSlotWords u32: the generation at [32] and two counter sets
 // at 256 and 2304, used by alternate barriers, so the last arrivers' resets
 // need no wait (a set is reused two barriers later, after the resetting
-// workgroup's next arrival has waited for its stores).  A set holds a top
+// workgroup's next arrival has waited for its stores).  Words 64..87 are
+// left to the co-simulation launch's flash state buffers.  A set holds a top
 // counter at [0] and, for n >= 128, eight sub-counters at [128 * (1 + x)],
 // x = member % 8 (per XCD when members are consecutive workgroups, which
 // are dispatched round-robin over the 8 XCDs): one contended atomic per
