@@ -90,6 +90,8 @@ def parse():
     ap.add_argument("--batched-groups", type=int, default=0,
                     help="lane groups of 32 slots (0 = one per 32 lanes)")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-cosim", action="store_true",
+                    help="skip the co-simulation edges reported under config.forms.cosim")
     ap.add_argument("--no-width-tune", action="store_true",
                     help="batched form with the compiler's default partition width "
                          "instead of the measured choice (batched.tune_partition_width)")
@@ -562,7 +564,7 @@ def bench_mult16m(a, torch):
     }), flush=True)
 
 
-def bench_cosim(a, torch):
+def bench_cosim(a, torch, comp=None, emit=True):
     """Reactive co-simulation edges per second.  (1) The SPI master with the
     flash model: 20,000 edges in batches of 1024, the first 400 edges checked
     against the CPU edge loop and the two bytes it reads against the image.
@@ -643,7 +645,8 @@ def bench_cosim(a, torch):
     # (2) the 1M-gate netlist clocked edge by edge
     import ctypes
     from jacquard_amd import _native as N
-    nl2, comp = build_design(a)
+    if comp is None:
+        _, comp = build_design(a)
     r = ctypes.c_int(0)
     N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(r)))
     s2 = comp.flatten(min(max(comp.parts_per_stage()), r.value))
@@ -671,6 +674,8 @@ def bench_cosim(a, torch):
                          "blocks": s2.num_blocks, "major_stages": s2.num_major_stages,
                          "cpu_edges_per_s": cpu2, "cpu_edges": 4}
     log(f"cosim 1M gates: {e2 / wall2:,.0f} edges/s ({wall2 / e2 * 1e6:.1f} us per edge)")
+    if not emit:
+        return res
     print(json.dumps({
         "metric": "co-simulation edges per second", "value": edges / wall,
         "unit": "edges/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -960,6 +965,9 @@ def main():
         log(f"batched: {forms['batched']['lane_cycles_per_s']:,.0f} lane-cycles/s "
             f"({forms['batched']['vs_plain']:.1f}x plain), "
             f"{forms['batched']['hbm']['achieved']:,.0f} GB/s algorithmic HBM traffic")
+    if not a.no_cosim:
+        # reactive co-simulation edges (the `--workload cosim` line's forms)
+        forms["cosim"] = bench_cosim(a, torch, comp=comp, emit=False)
     cpu = None
     if not a.no_cpu:
         c = cpu_baseline(a, script)
