@@ -579,3 +579,49 @@ def gpio_mapping(script, port_mapping=None):
         if pos is not None:
             output_bits[g] = pos
     return GpioMapping(input_bits, output_bits, domains, named)
+
+
+# ---- stimulus VCD ---------------------------------------------------------------
+
+def stimulus_vcd(script, trace, gcd_ps, first_edge=0, scope="top"):
+    """The co-simulation's stimulus VCD (setup_stimulus_vcd,
+    src/sim/vcd_io.rs:964-1017, and the per-edge writer of
+    src/sim/cosim_metal.rs:3587-3619): every input port bit of the netlist
+    with its value in each edge's input snapshot, at time edge * gcd_ps
+    (timescale 1 ps), only changed values after the first edge.  `trace`
+    is the uint32[n, 2*ss] trace of `Cosim.run(..., trace=True)`; edges are
+    numbered from `first_edge`.  Returns the VCD text."""
+    from .vcd import id_code
+    ins, _ = script.netlist.ports()
+    sig = []
+    for name, pin, _ in ins:
+        pos = script.input_map.get(pin)
+        if pos is None:
+            continue
+        base, index = name, None
+        if name.endswith("]") and "[" in name:
+            base, idx = name[:-1].split("[", 1)
+            if idx.isdigit():
+                index = int(idx)
+            else:
+                base = name
+        sig.append((pos, base, index))
+    lines = ["$timescale 1 ps $end", f"$scope module {scope} $end"]
+    for i, (_, base, index) in enumerate(sig):
+        ref = base if index is None else f"{base} [{index}]"
+        lines.append(f"$var wire 1 {id_code(i)} {ref} $end")
+    lines += ["$upscope $end", "$enddefinitions $end", "$dumpvars"]
+    lines += [f"0{id_code(i)}" for i in range(len(sig))]
+    lines.append("$end")
+    trace = np.asarray(trace, np.uint32)
+    prev = [None] * len(sig)
+    pos = np.asarray([p for p, _, _ in sig], np.int64)
+    for k in range(trace.shape[0]):
+        lines.append(f"#{(first_edge + k) * gcd_ps}")
+        vals = (trace[k, pos >> 5] >> (pos & 31).astype(np.uint32)) & 1 if len(sig) else []
+        for i, v in enumerate(vals):
+            v = int(v)
+            if v != prev[i]:
+                lines.append(f"{v}{id_code(i)}")
+                prev[i] = v
+    return "\n".join(lines) + "\n"

@@ -393,3 +393,36 @@ def test_cosim_runs_the_two_state_script_only():
     sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
     with pytest.raises(ValueError):
         cosim.Cosim(s, sched, [])
+
+
+def test_stimulus_vcd_replays_the_trace_inputs():
+    # reading the stimulus VCD back (value changes per timestamp) gives every
+    # input port bit of every edge's input snapshot
+    image = bytes(range(256)) * 2
+    s, pins, doms, inbit, outbit = spi_setup(0x10, image)
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
+    tr, _, _ = oracle_edges(s, pins, doms, inbit["rst"], image, 8, 60, sched)
+    text = cosim.stimulus_vcd(s, tr, sched.gcd_ps, first_edge=100)
+    head, body = text.split("$enddefinitions $end")
+    ids = {}
+    for line in head.splitlines():
+        if line.startswith("$var"):
+            f = line.split()
+            ids[f[3]] = " ".join(f[4:-1])
+    assert sorted(ids.values()) == ["clk", "miso", "rst"]
+    name2bit = {"clk": inbit["clk"], "rst": inbit["rst"], "miso": inbit["miso"]}
+    cur, t, seen = {}, None, 0
+    for tok in body.split():
+        if tok.startswith("#"):
+            if t is not None:
+                k = t // 5000 - 100
+                for code, name in ids.items():
+                    assert cur[code] == co._bit(tr[k], name2bit[name]), (k, name)
+                seen += 1
+            t = int(tok[1:])
+        elif tok[0] in "01":
+            cur[tok[1:]] = int(tok[0])
+    k = t // 5000 - 100
+    for code, name in ids.items():
+        assert cur[code] == co._bit(tr[k], name2bit[name])
+    assert seen + 1 == len(tr)
