@@ -264,10 +264,13 @@ class Cosim:
             if in_reset and self.edge + n > self.reset_edges:
                 n = self.reset_edges - self.edge
             reset_val = self.reset_active if in_reset else 1 - self.reset_active
-            self._upload_ops(reset_val)
-            if self.flash:
-                self.d_flash.view(torch.uint8)[_IN_RESET_OFFSET] = 1 if in_reset else 0
-            tr = torch.zeros((n, 2 * self.ss), dtype=torch.int32, device="cuda") if trace else None
+            # buffer updates are ordered with the launches on the same stream
+            with torch.cuda.stream(self.stream):
+                self._upload_ops(reset_val)
+                if self.flash:
+                    self.d_flash.view(torch.uint8)[_IN_RESET_OFFSET] = 1 if in_reset else 0
+                tr = (torch.zeros((n, 2 * self.ss), dtype=torch.int32, device="cuda")
+                      if trace else None)
             N.check(N.hip().jq_cosim_edges(
                 self.script.num_blocks, self.script.num_major_stages, self.d_bs.data_ptr(),
                 self.d_bd.data_ptr(), self.d_sram.data_ptr(), self.ss, self.states.data_ptr(),
