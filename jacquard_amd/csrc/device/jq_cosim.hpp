@@ -93,16 +93,9 @@ struct CosimArgs {
 };
 
 // Bit access inside one snapshot of ss words; positions past its end read
-// as 0 and are not written.
+// as 0.
 __device__ __forceinline__ u32 bit_at(const u32 *s, u32 ss, u32 pos) {
   return (pos >> 5) < ss ? (ld_coherent(s + (pos >> 5)) >> (pos & 31u)) & 1u : 0u;
-}
-
-__device__ __forceinline__ void put_bit(u32 *s, u32 ss, u32 pos, u32 v) {
-  if ((pos >> 5) >= ss) return;
-  const u32 m = 1u << (pos & 31u);
-  const u32 w = ld_coherent(s + (pos >> 5));
-  st_coherent(s + (pos >> 5), v ? (w | m) : (w & ~m));
 }
 
 // A byte completed on the SPI bus (csrc/spiflash_model.cc:40-101; the Metal
@@ -365,21 +358,42 @@ __device__ void between_phase(const CosimArgs &A, u32 t, u32 b, u32 nb, u32 e) {
     copy_words(tr + A.ss + lo, A.states + A.ss + lo, hi - lo, t, false);
   }
   if (last) return;
+  __shared__ u32 s_din;
+  if (t == 0) s_din = din;
   __syncthreads();
-  // state preparation (csrc/kernel_v1.metal:678-710): the output snapshot
-  // becomes the next input, then the edge's bit operations in order
-  copy_words(A.states + lo, A.states + A.ss + lo, hi - lo, t, true);
-  wait_vmem();
-  __syncthreads();
-  if (t == 0) {
-    const u32 pe = (u32)(((usize)A.sched_offset + e) % A.num_sched);
-    const u32 ob = A.op_start[pe], oe = A.op_start[pe + 1];
-    for (u32 k = ob; k < oe; ++k)
-      if (owns(A.ops[2 * k])) put_bit(A.states, A.ss, A.ops[2 * k], A.ops[2 * k + 1]);
-    // flash data-in (csrc/kernel_v1.metal:896-918): the model's d_i
-    if (flash && need_fs) {
-      for (u32 i = 0; i < 4; ++i)
-        if (owns(A.pins[i])) put_bit(A.states, A.ss, A.pins[i], (din >> i) & 1u);
+  // state preparation (csrc/kernel_v1.metal:678-710, 896-918): the output
+  // snapshot becomes the next input, with the edge's bit operations in order
+  // and then the flash's data-in bits applied to each word on its way
+  // (kCopyBatch agent-coherent loads in flight per thread)
+  const u32 pe = (u32)(((usize)A.sched_offset + e) % A.num_sched);
+  const u32 ob = A.op_start[pe], oe = A.op_start[pe + 1];
+  const u32 dbits = s_din;
+  for (u32 base = lo + t; base < hi; base += 256 * kCopyBatch) {
+    u32 v[kCopyBatch];
+#pragma unroll
+    for (u32 k = 0; k < kCopyBatch; ++k) {
+      const u32 i = base + 256 * k;
+      v[k] = i < hi ? ld_coherent(A.states + A.ss + i) : 0u;
+    }
+#pragma unroll
+    for (u32 k = 0; k < kCopyBatch; ++k) {
+      const u32 i = base + 256 * k;
+      if (i >= hi) continue;
+      u32 w = v[k];
+      for (u32 j = ob; j < oe; ++j) {
+        const u32 pos = A.ops[2 * j];
+        if ((pos >> 5) == i) {
+          const u32 m = 1u << (pos & 31u);
+          w = A.ops[2 * j + 1] ? (w | m) : (w & ~m);
+        }
+      }
+      if (flash)
+        for (u32 q = 0; q < 4; ++q)
+          if ((A.pins[q] >> 5) == i) {
+            const u32 m = 1u << (A.pins[q] & 31u);
+            w = (dbits >> q) & 1u ? (w | m) : (w & ~m);
+          }
+      st_coherent(A.states + i, w);
     }
   }
 }
