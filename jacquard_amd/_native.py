@@ -58,6 +58,7 @@ def host():
         "jqh_aig_in_port": (c_cp, [c_vp, c_u32, P(c_u32), P(c_u32)]),
         "jqh_aig_out_port": (c_cp, [c_vp, c_u32, P(c_u32)]),
         "jqh_aig_module": (c_cp, [c_vp]),
+        "jqh_aig_dff_name": (c_cp, [c_vp, c_u32]),
         "jqh_aig_set_delays": (None, [c_vp, c_vp]),
         "jqh_compile": (c_vp, [c_vp, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32]),
         "jqh_compiled_free": (None, [c_vp]),

@@ -60,6 +60,9 @@ struct Aig {
   std::vector<uint32_t> in_port_pins, in_port_flag;
   std::string module_name;
   std::vector<Dff> dffs;
+  // instance name of each DFF cell (netlists read from Verilog; empty for
+  // generated netlists)
+  std::vector<std::string> dff_names;
   std::vector<Sram> srams;
   // per-pin integer gate delay in ps (timing), empty when not loaded
   std::vector<uint16_t> delay_ps;

@@ -168,6 +168,11 @@ const char *jqh_aig_out_port(void *h, uint32_t i, uint32_t *lit) {
   return g->out_port_names[i].c_str();
 }
 const char *jqh_aig_module(void *h) { return static_cast<Aig *>(h)->module_name.c_str(); }
+// Instance name of DFF j ("" when the netlist carries none).
+const char *jqh_aig_dff_name(void *h, uint32_t j) {
+  auto *g = static_cast<Aig *>(h);
+  return j < g->dff_names.size() ? g->dff_names[j].c_str() : "";
+}
 
 void jqh_aig_set_delays(void *h, const uint16_t *delay) {
   auto *g = static_cast<Aig *>(h);

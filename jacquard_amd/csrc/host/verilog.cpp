@@ -63,7 +63,7 @@ struct Lexer {
 };
 
 struct Cell {
-  std::string type;
+  std::string type, name;
   std::map<std::string, std::string> pins;  // pin -> net bit name or const
 };
 
@@ -153,7 +153,7 @@ void read_aigpdk_verilog(const std::string &text, Aig &g) {
     // cell instance: TYPE NAME ( .PIN(net), ... );
     Cell c;
     c.type = t;
-    lx.next();  // instance name
+    c.name = lx.next();  // instance name
     lx.expect("(");
     for (;;) {
       std::string x = lx.next();
@@ -235,6 +235,7 @@ void read_aigpdk_verilog(const std::string &text, Aig &g) {
       auto q = c.pins.find("Q");
       if (q != c.pins.end()) net_lit[q->second] = d.q << 1;
       g.dffs.push_back(d);
+      g.dff_names.push_back(c.name);
       dff_cells.push_back(k);
     }
   }

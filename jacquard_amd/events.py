@@ -43,6 +43,49 @@ class PerWordSummary:
     worst_arrival_ps: Optional[int] = None
 
 
+class WordSymbolMap:
+    """State word -> the DFFs packed into it (src/flatten.rs:125-178,
+    1848-1879), for symbolic violation sites."""
+
+    def __init__(self, word_to_dffs):
+        # {word: [(name, bit)]} sorted by bit
+        self.word_to_dffs = {int(w): sorted(v, key=lambda d: d[1])
+                             for w, v in word_to_dffs.items()}
+
+    @classmethod
+    def from_script(cls, script):
+        """DFF j sits at the state bit its data input is written to
+        (output_map of its D literal); names are the netlist's DFF instance
+        names."""
+        A = script.netlist.arrays()
+        names = script.netlist.dff_names()
+        m = {}
+        for j, d in enumerate(A["dff_d"].tolist()):
+            pos = script.output_map.get(int(d))
+            if pos is None:
+                continue
+            name = names[j] if j < len(names) and names[j] else f"dff#{j}"
+            m.setdefault(pos // 32, []).append((name, pos % 32))
+        return cls(m)
+
+    def describe_word(self, word_id, max_names=4):
+        """'name[bit b], ... [+N more] [word=W]' (src/flatten.rs:150-170)."""
+        dffs = self.word_to_dffs.get(int(word_id))
+        if not dffs:
+            return f"<no DFF in word> [word={word_id}]"
+        shown = min(len(dffs), max(max_names, 1))
+        parts = [f"{n}[bit {b}]" for n, b in dffs[:shown]]
+        if len(dffs) > shown:
+            parts.append(f"+{len(dffs) - shown} more")
+        return f"{', '.join(parts)} [word={word_id}]"
+
+    def num_sites(self):
+        return sum(len(v) for v in self.word_to_dffs.values())
+
+    def num_words(self):
+        return len(self.word_to_dffs)
+
+
 def default_site(word_id):
     """Fallback site descriptor (src/event_buffer.rs:264-269)."""
     return f"word={word_id}"

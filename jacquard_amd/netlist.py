@@ -131,6 +131,13 @@ class Netlist:
             outs.append((name.decode(), lit.value))
         return ins, outs
 
+    def dff_names(self):
+        """Instance name of every DFF, in DFF order ("" where the netlist
+        has none, as for generated netlists)."""
+        L = N.host()
+        return [L.jqh_aig_dff_name(self._h, j).decode()
+                for j in range(self.counts()["num_dffs"])]
+
     def module_name(self):
         return N.host().jqh_aig_module(self._h).decode()
 
