@@ -199,15 +199,36 @@ def bench_plain(a, script, torch):
         out["verified_cycles"] = v if np.array_equal(got, ref) else -1
         if out["verified_cycles"] < 0:
             raise AssertionError("plain GPU run differs from the CPU oracle")
-        # and the last cycles of the run: the oracle continues from the GPU's
-        # snapshot `total - v` (every word of it, including DFF state) with
-        # the same inputs
-        tail = st[total - v:total + 1].cpu().numpy().view(np.uint32).reshape(-1)
-        ref = tail.copy()
-        ref.reshape(v + 1, ss)[1:, nw:] = 0
-        oracle.run(script, ref, v)
-        if not np.array_equal(tail, ref):
-            raise AssertionError("plain GPU run differs from the CPU oracle at its end")
+        if not script.sram_storage_size:
+            # and the last cycles of the run: the oracle continues from the
+            # GPU's snapshot `total - v` (every word of it, including DFF
+            # state) with the same inputs
+            tail = st[total - v:total + 1].cpu().numpy().view(np.uint32).reshape(-1)
+            ref = tail.copy()
+            ref.reshape(v + 1, ss)[1:, nw:] = 0
+            oracle.run(script, ref, v)
+            if not np.array_equal(tail, ref):
+                raise AssertionError("plain GPU run differs from the CPU oracle at its end")
+        else:
+            # SRAM contents at snapshot `total - v` are not kept, so both
+            # continue for v more cycles from the final snapshot and SRAM
+            sram_end = d_sram.cpu().numpy().view(np.uint32).copy()
+            ext = np.zeros((v + 1, ss), np.uint32)
+            ext[0] = st[total].cpu().numpy().view(np.uint32)
+            ext[1:, :nw] = stimulus.input_words(script, v, seed=a.seed + 1, p=a.toggle)[1:]
+            d_ext = torch.from_numpy(ext.reshape(-1).view(np.int32)).to("cuda")
+            d_sram2 = d_sram.clone()
+            N.check(L.jq_simulate_scan(
+                script.num_blocks, script.num_major_stages, d_bs.data_ptr(),
+                d_bd.data_ptr(), d_sram2.data_ptr(), d_sx.data_ptr(), v, ss,
+                d_ext.data_ptr(), None, None, 0, 0, ctypes.c_void_p(stream.cuda_stream)))
+            torch.cuda.synchronize()
+            ref = ext.reshape(-1).copy()
+            ref_sram = sram_end.copy()
+            oracle.run(script, ref, v, sram=ref_sram)
+            if not (np.array_equal(d_ext.cpu().numpy().view(np.uint32), ref) and
+                    np.array_equal(d_sram2.cpu().numpy().view(np.uint32), ref_sram)):
+                raise AssertionError("plain GPU run differs from the CPU oracle at its end")
         out["verified_cycles"] += v
     del st
     torch.cuda.empty_cache()
