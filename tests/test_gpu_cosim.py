@@ -119,3 +119,23 @@ def test_uart_and_bus_trace_match_oracle(blocks):
     hdr, entries = c.bus_entries()
     assert hdr.tolist() == [bus.write_head, cap, bus.current_tick, bus.prev_flags]
     assert entries.tolist() == [list(e) for e in bus.entries]
+
+
+def test_cosim_on_a_side_stream_matches_the_current_stream():
+    # buffer updates between batches (reset operations, the flash's reset
+    # flag) are ordered with the launches on the stream the run was given
+    import torch
+    image = bytes(range(256)) * 4
+    s, pins, doms, inbit, outbit = spi_setup(0x40, image)
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(5000)])
+    kw = dict(reset_pos=inbit["rst"], reset_active=1, reset_edges=10, flash_pins=pins,
+              flash_image=image)
+    print("cosim side stream: 300 edges in batches of 7", flush=True)
+    a = cosim.Cosim(s, sched, doms, **kw).run(300, batch=7, trace=True)
+    side = torch.cuda.Stream()
+    c = cosim.Cosim(s, sched, doms, stream=side, **kw)
+    b = c.run(300, batch=7, trace=True)
+    assert np.array_equal(a, b)
+    exp, _, fl = oracle_edges(s, pins, doms, inbit["rst"], image, 10, 300, sched)
+    assert np.array_equal(b, exp)
+    assert bytes(c.flash_state()) == fl.to_record().tobytes()
