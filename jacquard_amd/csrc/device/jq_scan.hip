@@ -442,43 +442,47 @@ __device__ void run_partition(const ScanArgs &A, const u32 *__restrict__ bd,
 
     // ---- SRAM + duplicate permutation, kernel_v1_impl.cuh:365-443
     const u32 nperm = num_srams * 4 + num_dups;
-    u32 pv = 0, px = 0;
-    if ((u32)t < nperm) {
-      pv = (gather_bits(wo_v, sd0, sd1, sd2, sd3) & ~sdf.y) ^ sdf.x;
-      if (is_x) px = gather_bits(wo_x, sd0, sd1, sd2, sd3) & ~sdf.y;
-    }
     pi += 5120;
-    const u32 en = __shfl(pv, (lane + 1) & 63);
-    const u32 wdata = __shfl(pv, (lane + 2) & 63);
-    const u32 wdata_x = __shfl(px, (lane + 2) & 63);
-    __syncthreads();  // every gather of wo_* is done before it is written
-    if ((u32)t < num_srams * 4) {
-      if ((t & 3) == 0) {
-        u32 s = (u32)t >> 2;
-        size_t st = (size_t)sram_offset + (size_t)s * kSramWords;
-        u32 ra = (pv & 0xffffu) & (kSramWords - 1);
-        u32 wa = (pv >> 16) & (kSramWords - 1);
-        u32 *ram = A.sram_data + st;
-        u32 rv = ram[ra], w0v = ram[wa];
-        ram[wa] = (w0v & ~en) | (wdata & en);
-        wo_v[num_ios - num_srams + s] = rv;
-        if (is_x) {
-          u32 rx = 0;
-          if (A.sram_xmask) {
-            u32 *ramx = A.sram_xmask + st;
-            rx = ramx[ra];
-            u32 w0x = ramx[wa];
-            ramx[wa] = (w0x & ~en) | (wdata_x & en);
-          }
-          wo_x[num_ios - num_srams + s] = rx;
-        }
+    // partitions without SRAMs or duplicate words skip the phase (and
+    // its two workgroup barriers); the condition is uniform
+    if (nperm) {
+      u32 pv = 0, px = 0;
+      if ((u32)t < nperm) {
+        pv = (gather_bits(wo_v, sd0, sd1, sd2, sd3) & ~sdf.y) ^ sdf.x;
+        if (is_x) px = gather_bits(wo_x, sd0, sd1, sd2, sd3) & ~sdf.y;
       }
-    } else if ((u32)t < nperm) {
-      u32 d = num_ios - num_srams - num_dups + ((u32)t - num_srams * 4);
-      wo_v[d] = pv;
-      if (is_x) wo_x[d] = px;
+      const u32 en = __shfl(pv, (lane + 1) & 63);
+      const u32 wdata = __shfl(pv, (lane + 2) & 63);
+      const u32 wdata_x = __shfl(px, (lane + 2) & 63);
+      __syncthreads();  // every gather of wo_* is done before it is written
+      if ((u32)t < num_srams * 4) {
+        if ((t & 3) == 0) {
+          u32 s = (u32)t >> 2;
+          size_t st = (size_t)sram_offset + (size_t)s * kSramWords;
+          u32 ra = (pv & 0xffffu) & (kSramWords - 1);
+          u32 wa = (pv >> 16) & (kSramWords - 1);
+          u32 *ram = A.sram_data + st;
+          u32 rv = ram[ra], w0v = ram[wa];
+          ram[wa] = (w0v & ~en) | (wdata & en);
+          wo_v[num_ios - num_srams + s] = rv;
+          if (is_x) {
+            u32 rx = 0;
+            if (A.sram_xmask) {
+              u32 *ramx = A.sram_xmask + st;
+              rx = ramx[ra];
+              u32 w0x = ramx[wa];
+              ramx[wa] = (w0x & ~en) | (wdata_x & en);
+            }
+            wo_x[num_ios - num_srams + s] = rx;
+          }
+        }
+      } else if ((u32)t < nperm) {
+        u32 d = num_ios - num_srams - num_dups + ((u32)t - num_srams * 4);
+        wo_v[d] = pv;
+        if (is_x) wo_x[d] = px;
+      }
+      __syncthreads();
     }
-    __syncthreads();
 
     // ---- clock-enable permutation + commit, kernel_v1_impl.cuh:446-558
     {
