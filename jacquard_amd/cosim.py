@@ -159,8 +159,12 @@ class FlashPins:
     d_in: tuple = (NC, NC, NC, NC)
 
     def words(self):
-        return np.asarray(list(self.d_in) + [self.clk_out, self.csn_out]
-                          + list(self.d_out) + [1], np.uint32)
+        """The 11 pin words of jq_cosim_edges (up to 4 data bits each way,
+        missing ones not connected)."""
+        d_in, d_out = list(self.d_in)[:4], list(self.d_out)[:4]
+        d_in += [NC] * (4 - len(d_in))
+        d_out += [NC] * (4 - len(d_out))
+        return np.asarray(d_in + [self.clk_out, self.csn_out] + d_out + [1], np.uint32)
 
 
 class Cosim:
