@@ -426,3 +426,10 @@ def test_stimulus_vcd_replays_the_trace_inputs():
     for code, name in ids.items():
         assert cur[code] == co._bit(tr[k], name2bit[name])
     assert seen + 1 == len(tr)
+
+
+def test_flash_pin_words_layout():
+    # d_in[4], clock, chip select, d_out[4], has_flash; short lists padded
+    w = cosim.FlashPins(clk_out=10, csn_out=11, d_out=(20,), d_in=(30, 31)).words()
+    NC = cosim.NC
+    assert w.tolist() == [30, 31, NC, NC, 10, 11, 20, NC, NC, NC, 1]
