@@ -227,7 +227,11 @@ int jq_lds_read_probe(u32 num_blocks, u32 iters, u32 *sink, void *stream);
  * uart_channels: the reference's UartDecoderState[4], UartParams and
  * UartChannel[4] (csrc/kernel_v1.metal:1027-1053), or NULL;
  * bus_trace / bus_trace_params: its WbTraceChannel (entries follow the
- * 16-byte header) and WbTraceParams (:1061-1101), or NULL.  All pointers
+ * 16-byte header) and WbTraceParams (:1061-1101), or NULL.
+ * timing_constraints / event_buffer / event_capacity / arrival_state_offset:
+ * as jq_simulate_scan, with state_size the effective size (values, then
+ * arrivals); events carry cycle 0, as in the reference's co-simulation
+ * dispatches (src/sim/cosim_metal.rs:591-600).  All pointers
  * are device pointers; the host keeps FlashState.in_reset and the reset bit
  * operations current between calls.  Bit positions past the snapshot are
  * ignored.  Asynchronous on `stream`. */
@@ -240,7 +244,9 @@ int jq_cosim_edges(usize num_blocks, usize num_major_stages,
                    const u32 *flash_pins, const u8 *flash_data,
                    u32 flash_data_size, void *uart_state, const void *uart_params,
                    void *uart_channels, void *bus_trace,
-                   const void *bus_trace_params, u32 *trace, void *stream);
+                   const void *bus_trace_params, const u32 *timing_constraints,
+                   u8 *event_buffer, u32 event_capacity, i32 arrival_state_offset,
+                   u32 *trace, void *stream);
 
 /* ---- Learned cost model MLP (no counterpart in the reference) -----------
  * Fully connected FP32 network in -> h1 -> h2 -> 1 with ReLU (in <= 32,

@@ -163,7 +163,7 @@ def hip():
         "jq_cosim_edges": (ctypes.c_int, [
             c_u64, c_u64, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_u32,
             c_u32, c_u32, c_vp, c_vp, c_vp, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp,
-            c_vp, c_vp]),
+            c_vp, c_vp, c_u32, ctypes.c_int32, c_vp, c_vp]),
         "jq_max_resident_blocks": (ctypes.c_int, [ctypes.c_int,
                                                   P(ctypes.c_int)]),
         "jq_mlp_forward": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int,

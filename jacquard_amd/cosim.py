@@ -173,12 +173,13 @@ class Cosim:
 
     def __init__(self, script, scheduler, domains, *, reset_pos=None, reset_active=1,
                  reset_edges=0, constants=(), model_driven=(), flash_pins=None,
-                 flash_image=b"", uarts=(), bus_trace=None, stream=None):
+                 flash_image=b"", uarts=(), bus_trace=None, constraints=None,
+                 event_capacity=1024, stream=None):
         import torch
         self.torch = torch
         self.script = script
-        if script.xprop_enabled or script.timing_arrivals_enabled:
-            raise ValueError("co-simulation runs the 2-state script")
+        if script.xprop_enabled:
+            raise ValueError("co-simulation runs the 2-state script (arrivals allowed)")
         self.ss = int(script.effective_state_size())
         self.sched = scheduler
         self.domains = list(domains)
@@ -214,6 +215,17 @@ class Cosim:
             self.d_image = torch.from_numpy(img.copy()).to(dev)
             self.image_size = len(flash_image)
         self._ops_reset_val = None
+        # timing: arrivals after the values, setup/hold checks against
+        # `constraints` ([clock_ps, per-word]) into an event buffer
+        self.arrival_offset = int(script.arrival_state_offset) \
+            if script.timing_arrivals_enabled else 0
+        self.timed = constraints is not None
+        if self.timed:
+            self.d_cons = torch.from_numpy(np.ascontiguousarray(constraints, np.uint32)
+                                           .view(np.int32)).to(dev)
+            self.event_capacity = int(event_capacity)
+            self.d_events = torch.zeros(4 + 8 * self.event_capacity, dtype=torch.int32,
+                                        device=dev)
         # UART decoders: [(tx output bit, edges per bit)], at most 4
         self.uarts = list(uarts)
         if len(self.uarts) > UART_MAX:
@@ -289,6 +301,9 @@ class Cosim:
                 self.d_uart_ch.data_ptr() if self.uarts else None,
                 self.d_bus.data_ptr() if self.bus else None,
                 self.d_bus_params.data_ptr() if self.bus else None,
+                self.d_cons.data_ptr() if self.timed else None,
+                self.d_events.data_ptr() if self.timed else None,
+                self.event_capacity if self.timed else 0, self.arrival_offset,
                 tr.data_ptr() if tr is not None else None,
                 ctypes.c_void_p(self.stream.cuda_stream)))
             if tr is not None:
@@ -310,6 +325,16 @@ class Cosim:
 
     def snapshots(self):
         return self.states.cpu().numpy().view(np.uint32).copy()
+
+    def events(self):
+        """(event records kept, total count) of every run so far."""
+        if not self.timed:
+            return None, 0
+        from .sim import EVENT_DTYPE
+        raw = self.d_events.cpu().numpy().view(np.uint32)
+        total = int(raw[0])
+        kept = min(total, self.event_capacity)
+        return raw[4:4 + 8 * kept].copy().view(EVENT_DTYPE), total
 
     def uart_bytes(self, i):
         """Bytes decoded by UART i so far (the last UART_CHANNEL_CAP when

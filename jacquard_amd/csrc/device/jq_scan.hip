@@ -591,13 +591,20 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
   }
 }
 // Persistent co-simulation: C.num_edges edges of one cycle each over the
-// two snapshots of C.states, every workgroup doing its share of the
+// two snapshots of C.states (timing events carry cycle 0, as every edge of
+// the reference's co-simulation dispatch does, src/sim/cosim_metal.rs:
+// 591-600), every workgroup doing its share of the
 // between-edges work of jq_cosim.hpp ahead of every cycle (and once after
 // the last).  A cycle
 // walks the partitions as scan_kernel does, except that the first
 // partition of a cycle is loaded after the barrier that follows the
 // preparation, which may change any input word.
+template <bool TIMING>
 __global__ __launch_bounds__(256) void cosim_kernel(ScanArgs A, cosim::CosimArgs C) {
+  if (A.timing) {
+    A.clock_ps = A.timing[0];
+    A.constraints = A.timing + 1;
+  }
   __shared__ u32 st_v[256], st_x[256], hin_v[256], hin_x[256];
   __shared__ u32 wo_v[256], wo_x[256];
   __shared__ u16 st_arr[256], wo_arr[256];
@@ -631,8 +638,8 @@ __global__ __launch_bounds__(256) void cosim_kernel(ScanArgs A, cosim::CosimArgs
         else if (!wrap)
           hn = load_hdr(bd, A.blocks_start[(size_t)(m + 1) * nb + b]);
         if (h.num_stages)
-          run_partition<false, false>(A, bd, h, P, in, out, e, 0u, st_v, st_x, hin_v, hin_x,
-                                      wo_v, wo_x, st_arr, wo_arr);
+          run_partition<TIMING, false>(A, bd, h, P, in, out, 0u, 0u, st_v, st_x, hin_v,
+                                       hin_x, wo_v, wo_x, st_arr, wo_arr);
         if (last) {
           wait_vmem();
           if (hn.num_stages) load_pre(P, bd, hn, in);
@@ -832,7 +839,10 @@ extern "C" int jq_cosim_edges(usize num_blocks, usize num_major_stages,
                               const u8 *flash_data, u32 flash_data_size,
                               void *uart_state, const void *uart_params,
                               void *uart_channels, void *bus_trace,
-                              const void *bus_trace_params, u32 *trace, void *stream) {
+                              const void *bus_trace_params,
+                              const u32 *timing_constraints, u8 *event_buffer,
+                              u32 event_capacity, i32 arrival_state_offset, u32 *trace,
+                              void *stream) {
   t_err.clear();
   if (num_edges == 0) return 0;
   if (uart_state && (!uart_params || !uart_channels))
@@ -854,6 +864,12 @@ extern "C" int jq_cosim_edges(usize num_blocks, usize num_major_stages,
   A.num_cycles = 1;
   A.state_size = state_size;
   A.states = states;
+  A.timing = timing_constraints;
+  A.events = reinterpret_cast<EvHdr *>(event_buffer);
+  A.ev_cap = event_capacity;
+  A.arrival_state_offset = arrival_state_offset;
+  if (A.timing && !A.events) return fail("cosim: timing constraints need an event buffer");
+  const bool timing = A.timing != nullptr || arrival_state_offset != 0;
   cosim::CosimArgs C{};
   C.states = states;
   C.ss = (u32)state_size;
@@ -880,8 +896,9 @@ extern "C" int jq_cosim_edges(usize num_blocks, usize num_major_stages,
   // slot, which the barrier does not use (generation at 32, counters from 256)
   C.fsbuf = A.bar + 64;
   void *args[] = {&A, &C};
-  hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&cosim_kernel),
-                                            dim3(A.num_blocks), dim3(256), args, 0,
+  const void *k = timing ? reinterpret_cast<const void *>(&cosim_kernel<true>)
+                        : reinterpret_cast<const void *>(&cosim_kernel<false>);
+  hipError_t e = hipLaunchCooperativeKernel(k, dim3(A.num_blocks), dim3(256), args, 0,
                                             (hipStream_t)stream);
   if (e != hipSuccess) return fail(std::string("cosim launch failed: ") + hipGetErrorString(e));
   return 0;

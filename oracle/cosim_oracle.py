@@ -243,11 +243,14 @@ def _put(words, pos, v):
 
 
 def run_edges(script, states, ops, op_start, sched_offset, num_edges, flash=None,
-              pins=None, uarts=(), bus=None):
+              pins=None, uarts=(), bus=None, constraints=None, events=None,
+              arrival_state_offset=0):
     """The edge loop over a two-snapshot buffer (uint32[2*ss], updated in
     place).  ops: uint32[n, 2] (state bit, value); op_start: CSR over the
-    schedule edges.  pins: 11 ints as jq_cosim_edges takes them.  Returns
-    the trace uint32[num_edges, 2*ss] (both snapshots after each edge)."""
+    schedule edges.  pins: 11 ints as jq_cosim_edges takes them.  With
+    timing, each edge's setup/hold events (cycle 0) are appended to the
+    list `events`.  Returns the trace uint32[num_edges, 2*ss] (both
+    snapshots after each edge)."""
     import oracle
     ss = states.size // 2
     num_sched = len(op_start) - 1
@@ -262,7 +265,15 @@ def run_edges(script, states, ops, op_start, sched_offset, num_edges, flash=None
         if use_flash:
             for i in range(4):
                 _put(inp, int(pins[i]), (flash.d_i >> i) & 1)
-        oracle.run(script, states, 1)
+        if constraints is not None or arrival_state_offset:
+            ev, cnt = oracle.run(script, states, 1, constraints=constraints,
+                                 arrival_state_offset=arrival_state_offset,
+                                 event_capacity=1 << 16)
+            if events is not None:
+                assert cnt <= 1 << 16
+                events.append(ev)
+        else:
+            oracle.run(script, states, 1)
         if use_flash:
             out = states[ss:]
             d_out = sum(_bit(out, int(pins[6 + i])) << i for i in range(4))

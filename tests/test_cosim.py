@@ -368,17 +368,18 @@ def test_cosim_entry_point_rejects_bad_arguments_before_any_device_work():
 
     def call(num_blocks=1, states=dummy, ss=4, ops=dummy, start=dummy, nsched=2,
              edges=3, fs=None, pins=None, img=None, img_size=0, us=None, up=None, uc=None,
-             bus=None, busp=None):
+             bus=None, busp=None, cons=None, events=None):
         return L.jq_cosim_edges(num_blocks, 1, dummy, dummy, dummy, ss, states, ops, start,
                                 nsched, 0, edges, fs, pins, img, img_size, us, up, uc, bus,
-                                busp, None, None)
+                                busp, cons, events, 0, 0, None, None)
 
     assert call(edges=0) == 0
     cases = [(dict(num_blocks=0), "num_blocks"), (dict(states=None), "state"),
              (dict(ss=0), "state"), (dict(nsched=0), "schedule"),
              (dict(start=None), "schedule"), (dict(fs=dummy), "flash pins"),
              (dict(img_size=16), "image"), (dict(us=dummy), "UART"),
-             (dict(us=dummy, up=dummy), "UART"), (dict(bus=dummy), "bus trace")]
+             (dict(us=dummy, up=dummy), "UART"), (dict(bus=dummy), "bus trace"),
+             (dict(cons=dummy), "event buffer")]
     for kw, word in cases:
         assert call(**kw) != 0, kw
         assert word in L.jq_last_error().decode(), (kw, L.jq_last_error())

@@ -139,3 +139,39 @@ def test_cosim_on_a_side_stream_matches_the_current_stream():
     exp, _, fl = oracle_edges(s, pins, doms, inbit["rst"], image, 10, 300, sched)
     assert np.array_equal(b, exp)
     assert bytes(c.flash_state()) == fl.to_record().tobytes()
+
+
+def test_cosim_with_timing_matches_oracle():
+    # gate delays, arrivals in the state and setup/hold checks on every edge
+    # (events carry cycle 0); every snapshot and the multiset of events
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    from jacquard_amd.sim import canonical_events
+    nl = Netlist.random(seed=5, gates=3000, depth=20, dffs=200, inputs=24, outputs=12,
+                        delay=(20, 200))
+    s = Compiled(nl, level_split=5).flatten(6)
+    s.inject_timing()
+    s.enable_timing_arrivals()
+    cons = s.build_timing_constraint_buffer(600, 50, 20)  # setup and hold events
+    doms = [d for _, d in cosim.clock_domains_of(s)]
+    sched = cosim.MultiClockScheduler([cosim.ClockTiming(750)])
+    edges = 120
+    print(f"cosim timing: {s.num_blocks} blocks, {s.num_major_stages} major stages, "
+          f"{edges} edges", flush=True)
+    c = cosim.Cosim(s, sched, doms, constraints=cons, event_capacity=1 << 16)
+    got = c.run(edges, batch=50, trace=True)
+    ops, start = [], [0]
+    for k in range(len(sched)):
+        ops += sched.edge_ops(k, doms)
+        start.append(len(ops))
+    st = np.zeros(2 * s.effective_state_size(), np.uint32)
+    evs = []
+    exp = co.run_edges(s, st, np.asarray(ops, np.uint32).reshape(-1, 2),
+                       np.asarray(start, np.uint32), 0, edges, constraints=cons, events=evs,
+                       arrival_state_offset=s.arrival_state_offset)
+    bad = np.nonzero((got != exp).any(axis=1))[0]
+    assert bad.size == 0, f"first differing edge {bad[:1]}"
+    ev, total = c.events()
+    want = np.concatenate(evs) if evs else ev[:0]
+    assert total == len(want) and set(want["event_type"].tolist()) == {4, 5}
+    assert canonical_events(ev).tobytes() == canonical_events(want).tobytes()
