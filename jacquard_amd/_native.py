@@ -88,6 +88,10 @@ def host():
         "jqh_vcd_last_error": (c_cp, []),
         "jqh_constraint_buffer": (None, [c_vp, c_u32, ctypes.c_uint16,
                                          ctypes.c_uint16, c_vp]),
+        "jqh_inject_delays": (c_u64, [c_vp, c_u64, c_vp, c_vp, c_vp, c_u64,
+                                      c_vp, c_vp, c_u64]),
+        "jqh_constraint_words": (c_u32, [c_u32, c_u64, c_vp, c_vp, c_vp, c_vp,
+                                         c_u64, c_vp]),
         "jqh_validate": (ctypes.c_int, [c_vp, c_u64, c_vp, c_u64, c_u64,
                                         c_u64, c_u64, c_u64, c_u64,
                                         c_i32]),

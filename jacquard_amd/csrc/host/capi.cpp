@@ -320,6 +320,26 @@ void jqh_constraint_buffer(void *h, uint32_t clock_ps, uint16_t setup, uint16_t 
   std::memcpy(out, v.data(), v.size() * 4);
 }
 
+// inject_timing_to_script's slot rule over caller-given arrays (see
+// inject_delays); returns the number of nonzero slots written.
+uint64_t jqh_inject_delays(uint32_t *bd, uint64_t n_bd, const uint64_t *offsets,
+                           const uint32_t *pin_start, const uint32_t *pins, uint64_t n_patch,
+                           const uint16_t *rise_ps, const uint16_t *fall_ps, uint64_t n_pins) {
+  return inject_delays(bd, n_bd, offsets, pin_start, pins, n_patch, rise_ps, fall_ps, n_pins);
+}
+
+// build_timing_constraint_buffer over per-DFF constraints; writes num_words
+// words to out and returns the saturated clock period.
+uint32_t jqh_constraint_words(uint32_t num_words, uint64_t clock_period_ps,
+                              const uint16_t *setup_ps, const uint16_t *hold_ps,
+                              const int16_t *clock_arrival_ps,
+                              const uint32_t *data_state_pos, uint64_t n, uint32_t *out) {
+  std::vector<DffConstraint> cs(n);
+  for (uint64_t i = 0; i < n; ++i)
+    cs[i] = {setup_ps[i], hold_ps[i], clock_arrival_ps[i], data_state_pos[i]};
+  return constraint_words(num_words, clock_period_ps, cs.data(), n, out);
+}
+
 // Walks every partition of a script exactly as the device interpreter does
 // and checks that every state, SRAM, LDS and script access stays in bounds
 // (the interpreter itself trusts the script, as the reference kernel does).

@@ -1450,31 +1450,73 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
   (void)c;
 }
 
+size_t inject_delays(uint32_t *bd, size_t n_bd, const uint64_t *offsets,
+                     const uint32_t *pin_start, const uint32_t *pins, size_t n_patch,
+                     const uint16_t *rise_ps, const uint16_t *fall_ps, size_t n_pins) {
+  if (n_pins == 0 || n_patch == 0) return 0;  // no delays or no slots: untouched
+  size_t patched = 0;
+  for (size_t k = 0; k < n_patch; ++k) {
+    uint32_t mx = 0;
+    for (uint32_t j = pin_start[k]; j < pin_start[k + 1]; ++j) {
+      const uint32_t pin = pins[j];
+      if (pin < n_pins) mx = std::max<uint32_t>(mx, std::max(rise_ps[pin], fall_ps[pin]));
+    }
+    const uint32_t d = std::min<uint32_t>(mx, 65535);
+    if (offsets[k] < n_bd) {
+      bd[offsets[k]] = d;
+      patched += d > 0;
+    }
+  }
+  return patched;
+}
+
 void inject_timing(const Aig &g, Script &s) {
   if (g.delay_ps.empty()) return;
+  std::vector<uint64_t> off;
+  std::vector<uint32_t> start{0}, pins;
   for (auto &pp : s.delay_patch) {
-    uint32_t mx = 0;
-    for (uint32_t pin : pp.second)
-      if (pin < g.delay_ps.size()) mx = std::max<uint32_t>(mx, g.delay_ps[pin]);
-    s.blocks_data[pp.first] = std::min<uint32_t>(mx, 65535);
+    off.push_back(pp.first);
+    pins.insert(pins.end(), pp.second.begin(), pp.second.end());
+    start.push_back((uint32_t)pins.size());
   }
+  inject_delays(s.blocks_data.data(), s.blocks_data.size(), off.data(), start.data(),
+                pins.data(), off.size(), g.delay_ps.data(), g.delay_ps.data(),
+                g.delay_ps.size());
   s.timing = true;
+}
+
+void effective_setup_hold(const DffConstraint &c, uint16_t *setup, uint16_t *hold) {
+  const int32_t su = (int32_t)c.setup_ps - (int32_t)c.clock_arrival_ps;
+  const int32_t ho = (int32_t)c.hold_ps + (int32_t)c.clock_arrival_ps;
+  *setup = (uint16_t)std::min<int32_t>(std::max<int32_t>(su, 0), 65535);
+  *hold = (uint16_t)std::min<int32_t>(std::max<int32_t>(ho, 0), 65535);
+}
+
+uint32_t constraint_words(uint32_t num_words, uint64_t clock_period_ps,
+                          const DffConstraint *c, size_t n, uint32_t *out) {
+  std::fill(out, out + num_words, 0u);
+  for (size_t i = 0; i < n; ++i) {
+    if (c[i].data_state_pos == kNone) continue;
+    const uint32_t w = c[i].data_state_pos / 32;
+    if (w >= num_words) continue;
+    uint16_t su, ho;
+    effective_setup_hold(c[i], &su, &ho);
+    // per word the tightest setup and hold; 0 means "none yet"
+    const uint16_t os = (uint16_t)(out[w] >> 16), oh = (uint16_t)(out[w] & 0xFFFF);
+    const uint16_t ns = os == 0 ? su : std::min(os, su);
+    const uint16_t nh = oh == 0 ? ho : std::min(oh, ho);
+    out[w] = (uint32_t)ns << 16 | nh;
+  }
+  return (uint32_t)std::min<uint64_t>(clock_period_ps, 0xFFFFFFFFull);
 }
 
 std::vector<uint32_t> constraint_buffer(const Script &s, uint32_t clock_ps,
                                         uint16_t setup_ps, uint16_t hold_ps) {
+  std::vector<DffConstraint> cs;
+  cs.reserve(s.dff_data_pos.size());
+  for (uint32_t pos : s.dff_data_pos) cs.push_back({setup_ps, hold_ps, 0, pos});
   std::vector<uint32_t> buf(1 + s.reg_io_state_size, 0);
-  buf[0] = clock_ps;
-  for (uint32_t pos : s.dff_data_pos) {
-    if (pos == kNone) continue;
-    uint32_t w = pos / 32;
-    if (w >= s.reg_io_state_size) continue;
-    uint32_t ex = buf[1 + w];
-    uint16_t os = (uint16_t)(ex >> 16), oh = (uint16_t)(ex & 0xFFFF);
-    uint16_t ns = os == 0 ? setup_ps : std::min(os, setup_ps);
-    uint16_t nh = oh == 0 ? hold_ps : std::min(oh, hold_ps);
-    buf[1 + w] = (uint32_t)ns << 16 | nh;
-  }
+  buf[0] = constraint_words(s.reg_io_state_size, clock_ps, cs.data(), cs.size(), buf.data() + 1);
   return buf;
 }
 

@@ -107,11 +107,36 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
              const std::vector<std::vector<double>> *costs, Script &s);
 
 // Fill delay slots from per-pin delays (inject_timing_to_script,
-// src/flatten.rs:1812-1846).
+// src/flatten.rs:1812-1846): slot k at blocks_data offset offsets[k] gets
+// the largest max(rise, fall) over its pins pins[pin_start[k] ..
+// pin_start[k+1]), clamped to 65535; pins past n_pins are skipped, offsets
+// past n_bd are not written, and with no delays or no slots nothing is
+// written.  Returns the number of nonzero slots written.
+size_t inject_delays(uint32_t *bd, size_t n_bd, const uint64_t *offsets,
+                     const uint32_t *pin_start, const uint32_t *pins, size_t n_patch,
+                     const uint16_t *rise_ps, const uint16_t *fall_ps, size_t n_pins);
 void inject_timing(const Aig &g, Script &s);
 
-// Per-state-word constraint words (build_timing_constraint_buffer,
-// src/flatten.rs:1891-1924).  Returns [clock_ps, words...].
+// One DFF's capture constraint (DFFConstraint, src/flatten.rs:77-95).
+struct DffConstraint {
+  uint16_t setup_ps, hold_ps;
+  int16_t clock_arrival_ps;  // capture clock arrival, folded into setup/hold
+  uint32_t data_state_pos;   // state bit of its D input, ~0 = none
+};
+
+// effective_setup_hold (src/flatten.rs:115-122): setup - arrival and
+// hold + arrival, each clamped to [0, 65535].
+void effective_setup_hold(const DffConstraint &c, uint16_t *setup, uint16_t *hold);
+
+// build_timing_constraint_buffer (src/flatten.rs:1891-1924): out[w] =
+// (min effective setup << 16) | min effective hold over the DFFs whose D
+// lands in state word w (0 = no DFF); constraints without a position or
+// past num_words are skipped.  Returns the clock period saturated to u32.
+uint32_t constraint_words(uint32_t num_words, uint64_t clock_period_ps,
+                          const DffConstraint *c, size_t n, uint32_t *out);
+
+// [clock_ps, words...] for a compiled script with one setup/hold pair for
+// every DFF.
 std::vector<uint32_t> constraint_buffer(const Script &s, uint32_t clock_ps,
                                         uint16_t setup_ps, uint16_t hold_ps);
 
