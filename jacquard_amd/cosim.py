@@ -181,6 +181,10 @@ class Cosim:
         if script.xprop_enabled:
             raise ValueError("co-simulation runs the 2-state script (arrivals allowed)")
         self.ss = int(script.effective_state_size())
+        script.validate(self.ss)
+        if constraints is not None and len(constraints) != 1 + script.reg_io_state_size:
+            raise ValueError("constraints must be [clock_ps] + one word per state word "
+                             f"({1 + script.reg_io_state_size} words)")
         self.sched = scheduler
         self.domains = list(domains)
         self.reset_pos = reset_pos
@@ -252,6 +256,9 @@ class Cosim:
             hdr = np.zeros(4 + 5 * cap, np.uint32)
             hdr[1] = cap
             self.d_bus = torch.from_numpy(hdr.view(np.int32)).to(dev)
+        # every buffer above was filled on the current stream; the launches
+        # go to self.stream, which must see those fills first
+        self.stream.wait_stream(torch.cuda.current_stream())
 
     def _upload_ops(self, reset_val):
         if reset_val == self._ops_reset_val:
@@ -327,14 +334,17 @@ class Cosim:
         return self.states.cpu().numpy().view(np.uint32).copy()
 
     def events(self):
-        """(event records kept, total count) of every run so far."""
+        """(event records kept, total count, overflow flag) of every run so
+        far; the flag is set once more events came than the capacity holds
+        (the event buffer header's second word, csrc/event_buffer.h:27-33)."""
         if not self.timed:
-            return None, 0
+            return None, 0, 0
         from .sim import EVENT_DTYPE
+        self.stream.synchronize()
         raw = self.d_events.cpu().numpy().view(np.uint32)
         total = int(raw[0])
         kept = min(total, self.event_capacity)
-        return raw[4:4 + 8 * kept].copy().view(EVENT_DTYPE), total
+        return raw[4:4 + 8 * kept].copy().view(EVENT_DTYPE), total, int(raw[1])
 
     def uart_bytes(self, i):
         """Bytes decoded by UART i so far (the last UART_CHANNEL_CAP when

@@ -621,11 +621,18 @@ __global__ __launch_bounds__(256) void cosim_kernel(ScanArgs A, cosim::CosimArgs
   const u32 *bd = A.blocks_data;
   const u32 *in = A.states;
   u32 *out = A.states + A.state_size;
+  // co-simulation snapshots are 2-state: a partition flagged X-capable runs
+  // its value lane only
+  auto hdr2 = [&](size_t off) {
+    PartHdr x = load_hdr(bd, off);
+    x.is_x = false;
+    return x;
+  };
   for (u32 e = 0; e <= C.num_edges; ++e) {
     cosim::between_phase(C, (u32)t, b, nb, e);
     if (e == C.num_edges) break;
     grid_sync_coherent(A.bar, nb, b, gen, &g_error);
-    PartHdr h = M ? load_hdr(bd, A.blocks_start[b]) : no_hdr();
+    PartHdr h = M ? hdr2(A.blocks_start[b]) : no_hdr();
     PartPre P;
     if (h.num_stages) load_pre(P, bd, h, in);
     for (u32 m = 0; m < M; ++m) {
@@ -634,9 +641,9 @@ __global__ __launch_bounds__(256) void cosim_kernel(ScanArgs A, cosim::CosimArgs
         const bool last = h.num_stages == 0 || h.is_last != 0;
         PartHdr hn = no_hdr();
         if (!last)
-          hn = load_hdr(bd, h.base + part_words(h));
+          hn = hdr2(h.base + part_words(h));
         else if (!wrap)
-          hn = load_hdr(bd, A.blocks_start[(size_t)(m + 1) * nb + b]);
+          hn = hdr2(A.blocks_start[(size_t)(m + 1) * nb + b]);
         if (h.num_stages)
           run_partition<TIMING, false>(A, bd, h, P, in, out, 0u, 0u, st_v, st_x, hin_v,
                                        hin_x, wo_v, wo_x, st_arr, wo_arr);
@@ -860,7 +867,7 @@ extern "C" int jq_cosim_edges(usize num_blocks, usize num_major_stages,
   A.blocks_start = blocks_start;
   A.blocks_data = blocks_data;
   A.sram_data = sram_data;
-  A.sram_xmask = sram_data;
+  A.sram_xmask = nullptr;  // 2-state: no SRAM X shadow
   A.num_cycles = 1;
   A.state_size = state_size;
   A.states = states;

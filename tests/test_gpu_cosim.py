@@ -141,12 +141,9 @@ def test_cosim_on_a_side_stream_matches_the_current_stream():
     assert bytes(c.flash_state()) == fl.to_record().tobytes()
 
 
-def test_cosim_with_timing_matches_oracle():
-    # gate delays, arrivals in the state and setup/hold checks on every edge
-    # (events carry cycle 0); every snapshot and the multiset of events
+def timed_setup():
     from jacquard_amd.netlist import Netlist
     from jacquard_amd.script import Compiled
-    from jacquard_amd.sim import canonical_events
     nl = Netlist.random(seed=5, gates=3000, depth=20, dffs=200, inputs=24, outputs=12,
                         delay=(20, 200))
     s = Compiled(nl, level_split=5).flatten(6)
@@ -155,11 +152,10 @@ def test_cosim_with_timing_matches_oracle():
     cons = s.build_timing_constraint_buffer(600, 50, 20)  # setup and hold events
     doms = [d for _, d in cosim.clock_domains_of(s)]
     sched = cosim.MultiClockScheduler([cosim.ClockTiming(750)])
-    edges = 120
-    print(f"cosim timing: {s.num_blocks} blocks, {s.num_major_stages} major stages, "
-          f"{edges} edges", flush=True)
-    c = cosim.Cosim(s, sched, doms, constraints=cons, event_capacity=1 << 16)
-    got = c.run(edges, batch=50, trace=True)
+    return s, cons, doms, sched
+
+
+def timed_oracle(s, cons, doms, sched, edges):
     ops, start = [], [0]
     for k in range(len(sched)):
         ops += sched.edge_ops(k, doms)
@@ -169,9 +165,44 @@ def test_cosim_with_timing_matches_oracle():
     exp = co.run_edges(s, st, np.asarray(ops, np.uint32).reshape(-1, 2),
                        np.asarray(start, np.uint32), 0, edges, constraints=cons, events=evs,
                        arrival_state_offset=s.arrival_state_offset)
+    return exp, (np.concatenate(evs) if evs else None)
+
+
+@pytest.mark.parametrize("capacity", [1 << 16, 16])
+@pytest.mark.parametrize("side_stream", [False, True])
+def test_cosim_with_timing_matches_oracle(capacity, side_stream):
+    # gate delays, arrivals in the state and setup/hold checks on every edge
+    # (events carry cycle 0); every snapshot, the total event count, the
+    # overflow flag and the records kept (all of them, or any `capacity` of
+    # the oracle's multiset); on a side stream with no flash the buffers the
+    # constructor fills must still reach the launches first
+    import torch
+    from jacquard_amd.sim import canonical_events
+    s, cons, doms, sched = timed_setup()
+    edges = 120
+    print(f"cosim timing: {s.num_blocks} blocks, {s.num_major_stages} major stages, "
+          f"{edges} edges, capacity {capacity}, side stream {side_stream}", flush=True)
+    c = cosim.Cosim(s, sched, doms, constraints=cons, event_capacity=capacity,
+                    stream=torch.cuda.Stream() if side_stream else None)
+    got = c.run(edges, batch=50, trace=True)
+    exp, want = timed_oracle(s, cons, doms, sched, edges)
     bad = np.nonzero((got != exp).any(axis=1))[0]
     assert bad.size == 0, f"first differing edge {bad[:1]}"
-    ev, total = c.events()
-    want = np.concatenate(evs) if evs else ev[:0]
-    assert total == len(want) and set(want["event_type"].tolist()) == {4, 5}
-    assert canonical_events(ev).tobytes() == canonical_events(want).tobytes()
+    ev, total, overflow = c.events()
+    assert want is not None and total == len(want)
+    assert set(want["event_type"].tolist()) == {4, 5}
+    assert overflow == int(total > capacity)
+    if total <= capacity:
+        assert canonical_events(ev).tobytes() == canonical_events(want).tobytes()
+    else:
+        assert len(ev) == capacity
+        pool = [r.tobytes() for r in want]
+        for r in ev:
+            pool.remove(r.tobytes())  # each kept record is one of the oracle's
+
+
+def test_cosim_rejects_a_short_constraint_buffer():
+    s, cons, doms, sched = timed_setup()
+    with pytest.raises(ValueError):
+        cosim.Cosim(s, sched, doms, constraints=cons[:-1])
+
