@@ -127,7 +127,10 @@ int jq_simulate_scan_profiled(usize num_blocks, usize num_major_stages,
  *   lane_info        : 2 u32 per lane, written: group*32 + slot, and the
  *                      group cycle at which the lane started (pre-set by the
  *                      caller for the lanes placed in slots)
- *   group_cycles     : num_groups u32, written: cycles each group ran
+ *   group_cycles     : num_groups u32, read: the group cycle each group
+ *                      starts from (0 for a fresh run, see
+ *                      jq_simulate_batched_chunk); written: the group
+ *                      cycle each group stopped at
  *   obs_bits[num_obs]: state bits captured after every group cycle into
  *                      obs_out[(g*obs_capacity + cycle)*num_obs + k]; lane
  *                      (g, l, start) sees its cycle c at group cycle start+c
@@ -160,6 +163,45 @@ int jq_simulate_batched_profiled(
     u32 *queue, u32 *group_cycles, const u32 *obs_bits, u32 num_obs,
     u32 obs_capacity, u32 *obs_out, u32 num_parts, u32 *samples,
     u32 sample_cycles, void *stream);
+
+/* Resumable form of jq_simulate_batched (no counterpart in the reference):
+ * the same launch, except that group_cycles is read as well as written --
+ * each group resumes at group_cycles[g] (0 for a fresh run), its slots'
+ * lanes at the start cycles recorded in lane_info -- and with
+ * max_group_cycles > 0 each group stops after that many group cycles (0:
+ * every lane runs to its end).  Successive launches over the same buffers
+ * therefore run one batched job in steps, and give the same lanes the
+ * same results as one launch. */
+int jq_simulate_batched_chunk(
+    usize num_groups, usize blocks_per_group, usize num_major_stages,
+    const usize *blocks_start, const u32 *blocks_data, const u32 *gather_table,
+    const u32 *gather_start, u32 reg_io_state_size, u32 sram_storage_size,
+    u32 *sram_data, u32 *states, const u32 *data_bits, u32 n_data,
+    const u32 *flag_bits, u32 n_flag, u32 n_lanes, const u32 *lane_cycles,
+    const u32 *lane_seed, const u32 *lane_thresh, u32 *lane_info, u32 *slots,
+    u32 *queue, u32 *group_cycles, u32 max_group_cycles, const u32 *obs_bits,
+    u32 num_obs, u32 obs_capacity, u32 *obs_out, void *stream);
+
+/* Batched lanes of recorded stimulus (no counterpart in the reference):
+ * as jq_simulate_batched_chunk, with every lane's inputs taken from packed
+ * input words instead of the hash.  Lane q (queue order) has
+ * lane_cycles[q] + 1 snapshots of in_words words at
+ * stimulus[stimulus_offset[q] + k * in_words], k = 0..lane_cycles[q]: the
+ * first in_words words of the snapshots that parse_input_vcd
+ * (src/sim/vcd_io.rs:347-455) writes for the lane's VCD, clock flags
+ * included.  The state bits in_bits[0..n_in) (every primary input and
+ * clock-flag bit) are set from snapshot k before cycle k - 1 runs; a lane
+ * entering a slot gets its snapshot 0.  The caller sets the snapshot-0
+ * input bits of the lanes placed in slots at start in `states`. */
+int jq_simulate_batched_stimulus(
+    usize num_groups, usize blocks_per_group, usize num_major_stages,
+    const usize *blocks_start, const u32 *blocks_data, const u32 *gather_table,
+    const u32 *gather_start, u32 reg_io_state_size, u32 sram_storage_size,
+    u32 *sram_data, u32 *states, const u32 *in_bits, u32 n_in, u32 in_words,
+    const u32 *stimulus, const usize *stimulus_offset, u32 n_lanes,
+    const u32 *lane_cycles, u32 *lane_info, u32 *slots, u32 *queue,
+    u32 *group_cycles, u32 max_group_cycles, const u32 *obs_bits, u32 num_obs,
+    u32 obs_capacity, u32 *obs_out, void *stream);
 
 /* Occupancy query used by the autotuner: the largest number of resident
  * workgroups (all co-resident, as the persistent launches require) for the

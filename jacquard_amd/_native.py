@@ -117,6 +117,8 @@ HIP_SYMBOLS = [
     "jq_simulate_scan_profiled",
     "jq_simulate_batched",
     "jq_simulate_batched_profiled",
+    "jq_simulate_batched_chunk",
+    "jq_simulate_batched_stimulus",
     "jq_max_resident_blocks",
     "jq_output_changes",
     "jq_lds_read_probe",
@@ -160,6 +162,14 @@ def hip():
             c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp, c_vp, c_vp, c_vp,
             c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp, c_u32, c_vp, c_u32,
             c_vp]),
+        "jq_simulate_batched_chunk": (ctypes.c_int, [
+            c_u64, c_u64, c_u64, c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp,
+            c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp, c_vp, c_vp, c_vp,
+            c_vp, c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp, c_vp]),
+        "jq_simulate_batched_stimulus": (ctypes.c_int, [
+            c_u64, c_u64, c_u64, c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp,
+            c_vp, c_vp, c_u32, c_u32, c_vp, c_vp, c_u32, c_vp, c_vp, c_vp,
+            c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp, c_vp]),
         "jq_output_changes": (ctypes.c_int, [c_vp, c_vp, c_u32, c_vp, c_u32,
                                              c_u32, c_vp, c_vp, c_vp, c_vp]),
         "jq_lds_read_probe": (ctypes.c_int, [c_u32, c_u32, c_vp, c_vp]),

@@ -98,10 +98,25 @@ class BatchedRun:
     with a mix of long and short lanes) and each slot takes the next queued
     lane when its lane completes.  `groups` defaults to one group per 32
     lanes; fewer groups than that make the queue hand out the remaining
-    lanes as slots free up."""
+    lanes as slots free up.
 
-    def __init__(self, script, seeds, probs, cycles, obs_lits=None, groups=None):
+    A lane's stimulus is either hashed on the device from (seed, toggle
+    probability) or, with `stimulus`, recorded: one uint32[cycles+1,
+    input_word_count] array per lane holding the input area of every
+    snapshot, as ``vcd.parse_input_vcd`` writes it (clock flags included).
+    ``launch(budget=k)`` runs k group cycles and returns; the next launch
+    resumes where it stopped."""
+
+    def __init__(self, script, seeds, probs, cycles, obs_lits=None, groups=None,
+                 stimulus=None):
         self.script = script
+        self.stim = None
+        if stimulus is not None:
+            nw = stimulus_word_count(script)
+            self.stim = [np.ascontiguousarray(a, np.uint32).reshape(-1, nw) for a in stimulus]
+            cycles = [len(a) - 1 for a in self.stim]
+            seeds = np.zeros(len(self.stim), np.uint32)
+            probs = np.zeros(len(self.stim))
         self.seeds = np.asarray(seeds, np.uint32)
         self.probs = np.asarray(probs, np.float64)
         self.cycles = np.asarray(cycles, np.int64)
@@ -144,10 +159,31 @@ class BatchedRun:
         s = self.script
         rio = s.reg_io_state_size
         st = np.zeros((self.G, 2, rio * 32), np.uint32)
-        st[:, :, self.flag_bits.astype(np.int64)] = 0xFFFFFFFF
-        lane_bd, table, start = batched_program(s)
         nz = lambda a: a if a.size else np.zeros(1, a.dtype)
-        self.d = dict(
+        extra = {}
+        if self.stim is None:
+            st[:, :, self.flag_bits.astype(np.int64)] = 0xFFFFFFFF
+        else:
+            # lanes in queue order, concatenated; snapshot-0 input bits of
+            # the lanes placed in slots go into the initial state
+            in_bits = np.concatenate([self.data_bits, self.flag_bits]).astype(np.uint32)
+            if in_bits.size and int(in_bits.max()) >= 32 * stimulus_word_count(s):
+                raise ValueError("an input bit lies outside the recorded input words")
+            q = [self.stim[i] for i in self.order.tolist()]
+            off = np.cumsum([0] + [a.size for a in q[:-1]]).astype(np.uint64)
+            for slot, k in enumerate(self.slots.tolist()):
+                if k == NO_SOURCE:
+                    continue
+                g, l = divmod(slot, 32)
+                w0 = q[k][0]
+                bits = (w0[in_bits >> 5] >> (in_bits & 31)) & 1
+                st[g, 0, in_bits.astype(np.int64)] |= bits.astype(np.uint32) << np.uint32(l)
+            extra = dict(in_bits=to_device(nz(in_bits)),
+                         stim=to_device(np.concatenate([a.reshape(-1) for a in q])),
+                         stim_off=to_device(nz(off)))
+            self.n_in = len(in_bits)
+        lane_bd, table, start = batched_program(s)
+        self.d = dict(extra,
             bs=to_device(np.ascontiguousarray(s.blocks_start, np.uint64)),
             bd=to_device(lane_bd),
             gtab=to_device(table),
@@ -210,10 +246,34 @@ class BatchedRun:
                 d["gcyc"].data_ptr(), d["obs_bits"].data_ptr(), len(self.obs_bits),
                 self.obs_cap, d["obs"].data_ptr()]
 
-    def launch(self, stream=None):
+    def launch(self, stream=None, budget=0):
+        """One launch: every lane to its end, or `budget` group cycles per
+        group (resuming where the previous launch stopped)."""
         import torch
         st = stream if stream is not None else torch.cuda.current_stream()
-        N.check(N.hip().jq_simulate_batched(*self._args(), ctypes.c_void_p(st.cuda_stream)))
+        sp = ctypes.c_void_p(st.cuda_stream)
+        if self.stim is not None:
+            s, d = self.script, self.d
+            N.check(N.hip().jq_simulate_batched_stimulus(
+                self.G, s.num_blocks, s.num_major_stages, d["bs"].data_ptr(),
+                d["bd"].data_ptr(), d["gtab"].data_ptr(), d["gstart"].data_ptr(),
+                s.reg_io_state_size, s.sram_storage_size, d["sram"].data_ptr(),
+                d["states"].data_ptr(), d["in_bits"].data_ptr(), self.n_in,
+                stimulus_word_count(s), d["stim"].data_ptr(), d["stim_off"].data_ptr(),
+                len(self.seeds), d["lcyc"].data_ptr(), d["info"].data_ptr(),
+                d["slots"].data_ptr(), d["queue"].data_ptr(), d["gcyc"].data_ptr(),
+                int(budget), d["obs_bits"].data_ptr(), len(self.obs_bits), self.obs_cap,
+                d["obs"].data_ptr(), sp))
+        elif budget:
+            a = self._args()
+            N.check(N.hip().jq_simulate_batched_chunk(*a[:23], int(budget), *a[23:], sp))
+        else:
+            N.check(N.hip().jq_simulate_batched(*self._args(), sp))
+
+    def done(self):
+        """True once every lane has run all its cycles (after a synchronise)."""
+        return bool((self.d["slots"].cpu().numpy().view(np.uint32) == NO_SOURCE).all()) and \
+            int(self.d["queue"].cpu().numpy().view(np.uint32)[0]) >= len(self.seeds)
 
     def launch_profiled(self, sample_cycles, stream=None):
         """Launch with per-partition clock samples of group 0 for the first
@@ -267,6 +327,12 @@ class BatchedRun:
             rows = obs[g, start:start + c, :].cpu().numpy().view(np.uint32)
             out[int(lane)] = ((rows >> np.uint32(l)) & 1).astype(np.uint8)
         return out
+
+
+def stimulus_word_count(script):
+    """Words of the snapshot's input area a recorded lane supplies per
+    snapshot (primary inputs and clock flags)."""
+    return stimulus.input_word_count(script)
 
 
 def plain_observations(script, states, cycles, obs_lits):

@@ -67,7 +67,15 @@ struct BArgs {
   u32 *lane_info;    // per lane: group*32 + slot, start cycle of the group
   u32 *slots;        // per group 32 slots: lane or kNoLane
   u32 *queue;        // next lane to hand out
-  u32 *group_cycles; // out: cycles each group ran
+  u32 *group_cycles; // in: group cycle to resume from; out: cycles each group ran
+  u32 budget;        // group cycles per launch (0: run every lane to the end)
+  // packed input words per lane (null: hashed stimulus): snapshot k of the
+  // lane in queue order q is stim[stim_off[q] + k*in_words ...], and the
+  // state bits in_bits[0..n_in) take their values from it
+  const u32 *stim;
+  const usize *stim_off;
+  const u32 *in_bits;
+  u32 n_in, in_words;
   const u32 *obs_bits;
   u32 n_obs, obs_cap;
   u32 *obs_out;
@@ -386,12 +394,18 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
   u32 *slots = A.slots + 32 * g;
   const u32 t = threadIdx.x, l = t & 31;
   const u32 par = b * 256 + t, npar = A.W * 256;  // group-wide thread index
-  // every thread mirrors slot l = t & 31 of its group
+  // every thread mirrors slot l = t & 31 of its group; a launch resumes
+  // the group at its recorded cycle, each slot's lane at the group cycle it
+  // started
+  const u32 c0 = A.group_cycles[g];
   u32 my_lane = slots[l], my_start = 0, my_end = 0, my_seed = 0, my_thr = 0;
   if (my_lane != kNoLane) {
-    my_end = A.lane_cycles[my_lane];
-    my_seed = A.lane_seed[my_lane];
-    my_thr = A.lane_thresh[my_lane];
+    my_start = A.lane_info[2 * my_lane + 1];
+    my_end = my_start + A.lane_cycles[my_lane];
+    if (!A.stim) {
+      my_seed = A.lane_seed[my_lane];
+      my_thr = A.lane_thresh[my_lane];
+    }
   }
   // header and gather entries of the block's first partition of major
   // stage 0
@@ -403,7 +417,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
     for (int j = 0; j < 32; ++j) src[j] = g0[t + 256 * j];
   }
   __syncthreads();
-  u32 c = 0;
+  u32 c = c0;
   for (;; ++c) {
     u32 *in = (c & 1) ? st1 : st0;
     u32 *out = (c & 1) ? st0 : st1;
@@ -418,6 +432,10 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
       u32 *o = A.obs_out + ((size_t)g * A.obs_cap + (c - 1)) * A.n_obs;
       for (u32 k = t; k < A.n_obs; k += 256) o[k] = ld_coherent(in + A.obs_bits[k]);
     }
+    // a launch with a budget stops after that many group cycles, with the
+    // observation of its last cycle written (the same decision in every
+    // block of the group)
+    if (A.budget && c - c0 == A.budget) break;
     // lanes that completed their cycles hand their slot to the next lane
     const bool fin = my_lane != kNoLane && c >= my_end;
     if (__syncthreads_or(fin)) {
@@ -436,8 +454,9 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
       // reset the finished slots' lane bit: state of this cycle's input,
       // clock flags set, the lane's SRAM words zeroed
       for (size_t w = par; w < snap; w += npar) st_coherent(in + w, ld_coherent(in + w) & ~finmask);
-      for (u32 k = par; k < A.n_flag; k += npar)
-        st_coherent(in + A.flag_bits[k], ld_coherent(in + A.flag_bits[k]) | finmask);
+      if (!A.stim)
+        for (u32 k = par; k < A.n_flag; k += npar)
+          st_coherent(in + A.flag_bits[k], ld_coherent(in + A.flag_bits[k]) | finmask);
       for (u32 m = finmask; m; m &= m - 1) {
         u32 *ram = ram_group + (size_t)(__ffs(m) - 1) * A.sram_size;
         for (u32 w = par; w < A.sram_size; w += npar) st_coherent(ram + w, 0u);
@@ -447,16 +466,53 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
         my_start = c;
         if (my_lane != kNoLane) {
           my_end = c + A.lane_cycles[my_lane];
-          my_seed = A.lane_seed[my_lane];
-          my_thr = A.lane_thresh[my_lane];
+          if (!A.stim) {
+            my_seed = A.lane_seed[my_lane];
+            my_thr = A.lane_thresh[my_lane];
+          }
         }
       }
       grid_sync_coherent(bar, A.W, b, gen, &gb_error);
+      if (A.stim) {
+        // packed stimulus: the new lanes' input bits of their snapshot 0
+        for (u32 k0 = b * 8; k0 < A.n_in; k0 += A.W * 8) {
+          const u32 k = k0 + (t >> 5);
+          bool bit = false;
+          if (k < A.n_in && fin && my_lane != kNoLane) {
+            const u32 p = A.in_bits[k];
+            bit = (A.stim[A.stim_off[my_lane] + (p >> 5)] >> (p & 31)) & 1u;
+          }
+          const unsigned long long m = __ballot(bit);
+          const u32 set = (u32)(m >> (32 * ((t >> 5) & 1)));
+          if (k < A.n_in && l == 0 && set) {
+            const u32 p = A.in_bits[k];
+            st_coherent(in + p, ld_coherent(in + p) | set);
+          }
+        }
+        grid_sync_coherent(bar, A.W, b, gen, &gb_error);
+      }
     }
     if (!__syncthreads_or(my_lane != kNoLane)) break;
+    if (A.stim) {
+      // stimulus of snapshot c+1 from the lanes' packed input words: input
+      // bit k of slot l is bit in_bits[k] of the lane's snapshot
+      // c - start + 1 (inactive slots read 0)
+      for (u32 k0 = b * 8; k0 < A.n_in; k0 += A.W * 8) {
+        const u32 k = k0 + (t >> 5);
+        bool bit = false;
+        if (k < A.n_in && my_lane != kNoLane) {
+          const u32 p = A.in_bits[k];
+          const size_t at = A.stim_off[my_lane] + (size_t)(c - my_start + 1) * A.in_words;
+          bit = (A.stim[at + (p >> 5)] >> (p & 31)) & 1u;
+        }
+        const unsigned long long m = __ballot(bit);
+        const u32 word = (u32)(m >> (32 * ((t >> 5) & 1)));
+        if (k < A.n_in && l == 0) st_coherent(out + A.in_bits[k], word);
+      }
+    }
     // stimulus of snapshot c+1: data bit k toggles for the lane in slot l
     // when hash(seed, lane cycle + 1, k) < thresh
-    for (u32 k0 = b * 8; k0 < A.n_data; k0 += A.W * 8) {
+    for (u32 k0 = b * 8; !A.stim && k0 < A.n_data; k0 += A.W * 8) {
       u32 k = k0 + (t >> 5);
       bool act = k < A.n_data && my_lane != kNoLane;
       u32 bit = 0;
@@ -544,8 +600,17 @@ int batched_impl(usize num_groups, usize blocks_per_group, usize num_major_stage
                  const u32 *lane_thresh, u32 *lane_info, u32 *slots, u32 *queue,
                  u32 *group_cycles, const u32 *obs_bits, u32 num_obs,
                  u32 obs_capacity, u32 *obs_out, u32 num_parts, u32 *samples,
-                 u32 sample_cycles, void *stream) {
+                 u32 sample_cycles, u32 budget, const u32 *stim, const usize *stim_off,
+                 const u32 *in_bits, u32 n_in, u32 in_words, void *stream) {
   set_error("");
+  if (stim) {
+    if (!stim_off || (n_in && !in_bits) || in_words == 0 ||
+        (usize)in_words > (usize)reg_io_state_size)
+      return set_error("batched launch: packed stimulus needs lane offsets, input bits "
+                       "and 0 < in_words <= reg_io_state_size");
+  } else if (!lane_seed || !lane_thresh) {
+    return set_error("batched launch: hashed stimulus needs lane seeds and thresholds");
+  }
   if (num_groups == 0 || blocks_per_group == 0) return 0;
   if (num_groups > (usize)kBatchedSlots) {
     return set_error("too many lane groups");
@@ -593,6 +658,12 @@ int batched_impl(usize num_groups, usize blocks_per_group, usize num_major_stage
   A.prof = samples;
   A.prof_cycles = samples ? sample_cycles : 0;
   A.n_parts = num_parts;
+  A.budget = budget;
+  A.stim = stim;
+  A.stim_off = stim_off;
+  A.in_bits = in_bits;
+  A.n_in = stim ? n_in : 0;
+  A.in_words = in_words;
   u32 *bar_base = nullptr;
   if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(gb_barrier)) != hipSuccess) {
     return set_error("hipGetSymbolAddress(gb_barrier) failed");
@@ -647,7 +718,44 @@ extern "C" int jq_simulate_batched(
                       sram_storage_size, sram_data, states, data_bits, n_data, flag_bits,
                       n_flag, n_lanes, lane_cycles, lane_seed, lane_thresh, lane_info,
                       slots, queue, group_cycles, obs_bits, num_obs, obs_capacity, obs_out,
-                      0, nullptr, 0, stream);
+                      0, nullptr, 0, 0, nullptr, nullptr, nullptr, 0, 0, stream);
+}
+
+extern "C" int jq_simulate_batched_chunk(
+    usize num_groups, usize blocks_per_group, usize num_major_stages,
+    const usize *blocks_start, const u32 *blocks_data, const u32 *gather_table,
+    const u32 *gather_start, u32 reg_io_state_size, u32 sram_storage_size,
+    u32 *sram_data, u32 *states, const u32 *data_bits, u32 n_data,
+    const u32 *flag_bits, u32 n_flag, u32 n_lanes, const u32 *lane_cycles,
+    const u32 *lane_seed, const u32 *lane_thresh, u32 *lane_info, u32 *slots,
+    u32 *queue, u32 *group_cycles, u32 max_group_cycles, const u32 *obs_bits,
+    u32 num_obs, u32 obs_capacity, u32 *obs_out, void *stream) {
+  return batched_impl(num_groups, blocks_per_group, num_major_stages, blocks_start,
+                      blocks_data, gather_table, gather_start, reg_io_state_size,
+                      sram_storage_size, sram_data, states, data_bits, n_data, flag_bits,
+                      n_flag, n_lanes, lane_cycles, lane_seed, lane_thresh, lane_info,
+                      slots, queue, group_cycles, obs_bits, num_obs, obs_capacity, obs_out,
+                      0, nullptr, 0, max_group_cycles, nullptr, nullptr, nullptr, 0, 0,
+                      stream);
+}
+
+extern "C" int jq_simulate_batched_stimulus(
+    usize num_groups, usize blocks_per_group, usize num_major_stages,
+    const usize *blocks_start, const u32 *blocks_data, const u32 *gather_table,
+    const u32 *gather_start, u32 reg_io_state_size, u32 sram_storage_size,
+    u32 *sram_data, u32 *states, const u32 *in_bits, u32 n_in, u32 in_words,
+    const u32 *stimulus, const usize *stimulus_offset, u32 n_lanes,
+    const u32 *lane_cycles, u32 *lane_info, u32 *slots, u32 *queue, u32 *group_cycles,
+    u32 max_group_cycles, const u32 *obs_bits, u32 num_obs, u32 obs_capacity,
+    u32 *obs_out, void *stream) {
+  if (!stimulus) return set_error("batched stimulus launch: stimulus missing");
+  return batched_impl(num_groups, blocks_per_group, num_major_stages, blocks_start,
+                      blocks_data, gather_table, gather_start, reg_io_state_size,
+                      sram_storage_size, sram_data, states, nullptr, 0, nullptr, 0,
+                      n_lanes, lane_cycles, nullptr, nullptr, lane_info, slots, queue,
+                      group_cycles, obs_bits, num_obs, obs_capacity, obs_out, 0, nullptr,
+                      0, max_group_cycles, stimulus, stimulus_offset, in_bits, n_in,
+                      in_words, stream);
 }
 
 extern "C" int jq_simulate_batched_profiled(
@@ -666,5 +774,6 @@ extern "C" int jq_simulate_batched_profiled(
                       sram_storage_size, sram_data, states, data_bits, n_data, flag_bits,
                       n_flag, n_lanes, lane_cycles, lane_seed, lane_thresh, lane_info,
                       slots, queue, group_cycles, obs_bits, num_obs, obs_capacity, obs_out,
-                      num_parts, samples, sample_cycles, stream);
+                      num_parts, samples, sample_cycles, 0, nullptr, nullptr, nullptr, 0, 0,
+                      stream);
 }

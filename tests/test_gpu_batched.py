@@ -142,3 +142,96 @@ def test_lane_vcd_matches_plain_path_vcd():
         oracle.run(s, st, c)
         ref = vcd.write_output_vcd(s, [((k + 1) * rio, ts[k]) for k in range(c)], st)
         assert got == ref, f"lane {lane}"
+
+
+def test_batched_chunks_equal_one_launch():
+    """The batched job run as launches of a few group cycles each (every
+    launch resumes the groups, slots and lanes where the previous one
+    stopped) gives every lane the outputs of the single launch."""
+    import torch
+    nl, comp = design("sram", level_split=8, seed=4, gates=30000, depth=48,
+                      dffs=3000, inputs=64, outputs=32, geometric_p=0.3, srams=4)
+    s = comp.flatten(16)
+    seeds, probs, cycles = _lanes(48, 10, 50, 5)
+    one = BatchedRun(s, seeds, probs, cycles, groups=1)
+    ref = one.run()
+    run = BatchedRun(s, seeds, probs, cycles, groups=1)
+    run.prepare()
+    torch.cuda.synchronize()
+    launches = 0
+    while not run.done():
+        run.launch(budget=7)
+        torch.cuda.synchronize()
+        launches += 1
+        assert launches < 200
+    print("launches", launches, "group cycles", run.group_cycles().tolist(), flush=True)
+    assert launches > 5
+    got = run.lane_outputs()
+    for lane in range(48):
+        assert np.array_equal(got[lane], ref[lane]), f"lane {lane}"
+    _check(s, one, seeds, probs, cycles, [0, 5, 47])
+
+
+def random_input_vcd(rng, cycles, data_names, clock="clk", half=5):
+    """An input VCD of `cycles` clock periods: the clock toggles every
+    `half` time units from 0; each data input changes at a random time
+    inside a period with probability 0.3 (sometimes at a clock edge's
+    timestamp, which the reader applies before that edge)."""
+    codes = {n: f"!{i}" if i < 90 else f"#{i}" for i, n in enumerate(data_names)}
+    ck = "%c"
+    lines = ["$timescale 1ps $end", "$scope module top $end",
+             f"$var wire 1 {ck} {clock} $end"]
+    for n in data_names:
+        base, idx = n[:-1].split("[")
+        lines.append(f"$var wire 1 {codes[n]} {base} [{idx}] $end")
+    lines += ["$upscope $end", "$enddefinitions $end", "$dumpvars", f"0{ck}"]
+    lines += [f"0{codes[n]}" for n in data_names] + ["$end"]
+    events = {}
+    for c in range(cycles):
+        events.setdefault(2 * c * half, []).append(f"1{ck}")
+        events.setdefault((2 * c + 1) * half, []).append(f"0{ck}")
+        for n in data_names:
+            if rng.random() < 0.3:
+                t = 2 * c * half + int(rng.integers(0, 2 * half))
+                events.setdefault(t, []).append(f"{int(rng.integers(0, 2))}{codes[n]}")
+    for t in sorted(events):
+        lines.append(f"#{t}")
+        lines += events[t]
+    lines.append(f"#{2 * cycles * half}")
+    lines.append(f"1{ck}")
+    return "\n".join(lines) + "\n"
+
+
+def test_batched_lanes_from_input_vcds():
+    """Lanes whose stimulus is read from input VCDs (parse_input_vcd's input
+    words, clock flags included) run through the batched form and the lane
+    queue; every lane's output VCD is the text the plain drop-in writes for
+    the same VCD, and equals the CPU oracle's."""
+    from jacquard_amd import sim, vcd
+    from jacquard_amd.batched import stimulus_word_count
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    s = comp.flatten(8)
+    ins, _ = nl.ports()
+    names = [n for (n, _, f) in ins if not f]
+    rng = np.random.default_rng(17)
+    nw = stimulus_word_count(s)
+    rio = s.reg_io_state_size
+    lanes = []
+    for k in range(40):
+        text = random_input_vcd(rng, int(rng.integers(8, 60)), names)
+        states, offs, _ = vcd.parse_input_vcd(text, s)
+        lanes.append((states, offs))
+    stim = [st.reshape(-1, rio)[:, :nw] for st, _ in lanes]
+    run = BatchedRun(s, None, None, None, stimulus=stim, groups=1)
+    outs = run.run()
+    for k in (0, 3, 17, 31, 32, 39):
+        states, offs = lanes[k]
+        c = len(offs)
+        ref = states.copy()
+        oracle.run(s, ref, c)
+        plain = sim.run_scan(s, states.copy(), c)["states"]
+        assert np.array_equal(plain, ref)
+        want = vcd.write_output_vcd(s, offs, plain)
+        rows = [i for i, (_, ts) in enumerate(offs) if ts != vcd.U64_MAX]
+        got = vcd.write_lane_vcd(s, outs[k][rows], run.obs_lits, [offs[i][1] for i in rows])
+        assert got == want, f"lane {k}"
