@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _native as N
-from . import stimulus
+from . import stimulus as _stimulus
 
 
 NO_SOURCE = 0xFFFFFFFF
@@ -134,7 +134,7 @@ class BatchedRun:
         self.G = groups or max(1, (n + 31) // 32)
         self.q_cycles = self.cycles[self.order].astype(np.uint32)
         self.q_seed = self.seeds[self.order]
-        self.q_thr = np.asarray([stimulus.threshold(self.probs[i]) for i in self.order.tolist()],
+        self.q_thr = np.asarray([_stimulus.threshold(self.probs[i]) for i in self.order.tolist()],
                                 np.uint32)
         self.slots = np.full(self.G * 32, NO_SOURCE, np.uint32)
         self.lane_info = np.zeros(2 * max(n, 1), np.uint32)
@@ -147,7 +147,7 @@ class BatchedRun:
         total = int(self.cycles.sum())
         self.max_cycles = int(self.cycles.max()) if n else 0
         self.obs_cap = -(-total // (32 * self.G)) + self.max_cycles + 1
-        self.data_bits, self.flag_bits = stimulus.input_roles(script)
+        self.data_bits, self.flag_bits = _stimulus.input_roles(script)
 
     def lane_cycles_total(self):
         return int(self.cycles.sum())
@@ -332,7 +332,7 @@ class BatchedRun:
 def stimulus_word_count(script):
     """Words of the snapshot's input area a recorded lane supplies per
     snapshot (primary inputs and clock flags)."""
-    return stimulus.input_word_count(script)
+    return _stimulus.input_word_count(script)
 
 
 def plain_observations(script, states, cycles, obs_lits):
