@@ -307,6 +307,22 @@ int jq_mlp_train_step(float *params, float *grads, float *workspace, int in,
                       int h1, int h2, const float *x, const float *target,
                       int n, float lr, float *loss_out, void *stream);
 
+/* Allocation-free train step (no counterpart in the reference; for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight
+ * training loops, harnesses that forbid hipMalloc).  Same computation and
+ * results as jq_mlp_train_step, with the workspace taken from a static
+ * device array of the library (1.6 MB per GPU, held until the process
+ * ends) instead of an argument.  n may not exceed
+ * jq_mlp_noalloc_max_samples() (4096); larger batches return an error.
+ * Calls, and the graph replays that contain them, share one workspace per
+ * GPU: calls that overlap on one GPU (different streams, no ordering) race
+ * and give wrong gradients.  Nothing in the workspace needs clearing: every
+ * call writes all of it that it reads. */
+size_t jq_mlp_noalloc_max_samples(void);
+int jq_mlp_train_step_noalloc(float *params, float *grads, int in, int h1,
+                              int h2, const float *x, const float *target,
+                              int n, float lr, float *loss_out, void *stream);
+
 /* Last error message of this thread ("" if none). */
 const char *jq_last_error(void);
 

@@ -129,6 +129,8 @@ HIP_SYMBOLS = [
     "jq_mlp_train_step",
     "jq_mlp_num_params",
     "jq_mlp_workspace_floats",
+    "jq_mlp_noalloc_max_samples",
+    "jq_mlp_train_step_noalloc",
     "jq_last_error",
 ]
 
@@ -189,6 +191,11 @@ def hip():
                                              ctypes.c_float, c_vp, c_vp]),
         "jq_mlp_num_params": (ctypes.c_size_t, [ctypes.c_int] * 3),
         "jq_mlp_workspace_floats": (ctypes.c_size_t, [ctypes.c_int] * 4),
+        "jq_mlp_noalloc_max_samples": (ctypes.c_size_t, []),
+        "jq_mlp_train_step_noalloc": (ctypes.c_int, [c_vp, c_vp, ctypes.c_int,
+                                                     ctypes.c_int, ctypes.c_int,
+                                                     c_vp, c_vp, ctypes.c_int,
+                                                     ctypes.c_float, c_vp, c_vp]),
         "jq_last_error": (c_cp, []),
         "jq_device_error_flag": (ctypes.c_int, [P(c_u32), ctypes.c_int]),
     }

@@ -9,12 +9,25 @@
 // not worth MFMA tiles.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "jq_common.hpp"
 
 namespace jqd {
 namespace {
 
 constexpr int kMaxIn = 32, kMaxH = 64, kSamples = 64;
+
+// Workspace of jq_mlp_train_step_noalloc: the per-workgroup gradient rows
+// and losses of a batch of at most kNoAllocMaxSamples samples at the largest
+// dimensions (64 rows of 6337 parameters + 64 losses, 1.6 MB).  Every GPU
+// the library runs on holds its own copy until the process ends.
+constexpr int kNoAllocMaxSamples = 4096;
+constexpr size_t kNoAllocFloats =
+    (size_t)(kNoAllocMaxSamples / kSamples) *
+        (kMaxH * kMaxIn + kMaxH + kMaxH * kMaxH + kMaxH + kMaxH + 1) +
+    kNoAllocMaxSamples / kSamples;
+__device__ __align__(256) float scratch[kNoAllocFloats];
 
 struct Dims {
   int in, h1, h2, np;
@@ -204,6 +217,7 @@ extern "C" int jq_mlp_train_step(float *params, float *grads, float *workspace,
   if (check_dims(in, h1, h2)) return -1;
   set_error("");
   if (n <= 0) return 0;
+  if (!workspace) return set_error("MLP train step: workspace missing");
   Dims d = dims(in, h1, h2);
   int nblk = (n + kSamples - 1) / kSamples;
   size_t lds = grad_lds_bytes(d);
@@ -223,4 +237,52 @@ extern "C" int jq_mlp_train_step(float *params, float *grads, float *workspace,
                      loss_out);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_error(hipGetErrorString(e));
+}
+
+namespace jqd {
+namespace {
+
+// Device address of `scratch` on each device, looked up once per device
+// with that device current (the array's host-side name is not a device
+// address).
+constexpr int kMaxDevices = 64;
+float *g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_mu;
+
+int scratch_for_current_device(float **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return set_error("MLP train step (no-alloc): no usable current device");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error("MLP train step (no-alloc): hipGetSymbolAddress(scratch) failed");
+    g_scratch_ptr[dev] = static_cast<float *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+}  // namespace jqd
+
+extern "C" size_t jq_mlp_noalloc_max_samples(void) { return (size_t)kNoAllocMaxSamples; }
+
+extern "C" int jq_mlp_train_step_noalloc(float *params, float *grads, int in, int h1,
+                                         int h2, const float *x, const float *target,
+                                         int n, float lr, float *loss_out,
+                                         void *stream) {
+  if (check_dims(in, h1, h2)) return -1;
+  if (n > kNoAllocMaxSamples)
+    return set_error("MLP train step (no-alloc): batch larger than 4096 samples");
+  set_error("");
+  if (n <= 0) return 0;
+  float *ws = nullptr;
+  if (scratch_for_current_device(&ws)) return -1;
+  if (jq_mlp_workspace_floats(in, h1, h2, n) > kNoAllocFloats)
+    return set_error("MLP train step (no-alloc): workspace exceeds the static buffer");
+  return jq_mlp_train_step(params, grads, ws, in, h1, h2, x, target, n, lr, loss_out,
+                           stream);
 }
