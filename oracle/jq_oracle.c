@@ -716,7 +716,8 @@ static inline vx_t lit(const vx_t *val, u32 iv) {
 int jqo_aig_sim(const jqo_aig *g, size_t num_cycles, size_t num_inputs,
                 const u32 *in_pins, const uint8_t *in_bits, size_t num_obs,
                 const u32 *obs_pins, uint8_t *obs_out, int xprop,
-                const uint8_t *dff_init_x, uint8_t *reg_out) {
+                const uint8_t *dff_init_x, const uint8_t *dff_init_v,
+                uint8_t *reg_out) {
   size_t n = (size_t)g->num_pins + 1;
   vx_t *val = (vx_t *)calloc(n, sizeof(vx_t));
   vx_t *reg = (vx_t *)calloc(g->num_dffs ? g->num_dffs : 1, sizeof(vx_t));
@@ -732,8 +733,10 @@ int jqo_aig_sim(const jqo_aig *g, size_t num_cycles, size_t num_inputs,
                                        : 1);
        ++i)
     memx[i] = xprop ? 0xFFFFFFFFu : 0;
-  for (u32 d = 0; d < g->num_dffs; ++d)
+  for (u32 d = 0; d < g->num_dffs; ++d) {
     reg[d].x = (uint8_t)(xprop && dff_init_x && dff_init_x[d]);
+    reg[d].v = (uint8_t)(dff_init_v ? dff_init_v[d] & 1 : 0);
+  }
   /* SRAM read-data registers start known, as the reference's X template
    * clears every input_map bit (src/sim/vcd_io.rs:470-474); the memory
    * array itself starts all X (src/bin/jacquard.rs:1293-1295). */

@@ -126,13 +126,17 @@ typedef struct {
  *   in_bits  : num_cycles * num_inputs input values (0/1), in_pins order
  *   obs_pins : pins whose combinational value is recorded every cycle
  *   obs_out  : num_cycles * num_obs values (bit0 value, bit1 X)
- *   dff_init_x : per-DFF initial X flag (NULL = all known-0)
+ *   dff_init_x : per-DFF initial X flag (NULL = all known)
+ *   dff_init_v : per-DFF initial value, bit 0 (NULL = all 0); with it a run
+ *                can start from any register state, e.g. a snapshot taken
+ *                in the middle of a long simulation
  *   reg_out  : if non-NULL, num_cycles * num_dffs DFF values after commit
  * Returns 0. */
 int jqo_aig_sim(const jqo_aig *g, size_t num_cycles, size_t num_inputs,
                 const uint32_t *in_pins, const uint8_t *in_bits,
                 size_t num_obs, const uint32_t *obs_pins, uint8_t *obs_out,
-                int xprop, const uint8_t *dff_init_x, uint8_t *reg_out);
+                int xprop, const uint8_t *dff_init_x, const uint8_t *dff_init_v,
+                uint8_t *reg_out);
 
 #ifdef __cplusplus
 }

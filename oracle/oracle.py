@@ -42,7 +42,7 @@ def lib():
                           vp, sz, vp]
     L.jqo_run_v1.argtypes = [sz, sz, vp, vp, vp, sz, sz, vp]
     L.jqo_run_v1_parallel.argtypes = [sz, sz, vp, vp, vp, sz, sz, vp, ctypes.c_int]
-    L.jqo_aig_sim.argtypes = [vp, sz, sz, vp, vp, sz, vp, vp, ctypes.c_int, vp, vp]
+    L.jqo_aig_sim.argtypes = [vp, sz, sz, vp, vp, sz, vp, vp, ctypes.c_int, vp, vp, vp]
     for f in ("jqo_simulate_block_v1", "jqo_simulate_block_v1_xprop",
               "jqo_block_kernel", "jqo_run", "jqo_run_v1", "jqo_run_v1_parallel",
               "jqo_aig_sim"):
@@ -121,9 +121,11 @@ class _Aig(ctypes.Structure):
 
 
 def aig_sim(arrays, num_cycles, in_pins, in_bits, obs_lits, xprop=False,
-            dff_init_x=None, want_regs=False):
-    """Direct cycle simulation of the AIG (no script).  Returns
-    (obs uint8[cycles, n_obs] with bit0 value / bit1 X, regs or None)."""
+            dff_init_x=None, want_regs=False, dff_init=None):
+    """Direct cycle simulation of the AIG (no script), from all-zero
+    registers or from the register values `dff_init` (uint8 per DFF).
+    Returns (obs uint8[cycles, n_obs] with bit0 value / bit1 X, regs or
+    None)."""
     A = arrays
     g = _Aig(A["drv"].size - 1, A["drv"].ctypes.data, A["and_a"].ctypes.data,
              A["and_b"].ctypes.data, A["dff_d"].size, A["dff_d"].ctypes.data,
@@ -137,9 +139,11 @@ def aig_sim(arrays, num_cycles, in_pins, in_bits, obs_lits, xprop=False,
     out = np.zeros((num_cycles, obs.size), np.uint8)
     regs = np.zeros((num_cycles, A["dff_d"].size), np.uint8) if want_regs else None
     dx = None if dff_init_x is None else np.ascontiguousarray(dff_init_x, np.uint8)
+    dv = None if dff_init is None else np.ascontiguousarray(dff_init, np.uint8)
+    assert dv is None or dv.size == A["dff_d"].size
     rc = lib().jqo_aig_sim(ctypes.byref(g), num_cycles, in_pins.size, _p(in_pins),
                            _p(in_bits), obs.size, _p(obs), _p(out), int(xprop),
-                           _p(dx), _p(regs))
+                           _p(dx), _p(dv), _p(regs))
     if rc:
         raise RuntimeError("aig_sim failed")
     return out, regs

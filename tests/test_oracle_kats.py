@@ -184,3 +184,31 @@ def test_hold_slack(arr, hold, slack):
 def test_zero_constraint_never_fires():
     assert len(timing_script(0, 0, 1)) == 0
     assert len(timing_script(5000, 0, 1)) == 0
+
+
+def test_aig_sim_resumes_from_register_values():
+    """A direct AIG run started from the registers after cycle N-1 of a
+    longer run (and the inputs from cycle N on) reproduces that run's
+    outputs and registers from cycle N: the property the windowed checks of
+    long GPU runs rely on."""
+    from helpers import CONFIG1, design
+    from jacquard_amd import stimulus
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    s = comp.flatten(8)
+    A = nl.arrays()
+    data, fl = stimulus.input_roles(s)
+    cycles, n = 60, 25
+    vals = stimulus.lane_inputs(len(data), cycles, 4, 0.4)
+    pos2pin = {v: k for k, v in s.input_map.items() if A["drv"][k] in (2, 3)}
+    pins = [pos2pin[p] for p in data.tolist()] + [pos2pin[p] for p in fl.tolist()]
+    bits = np.concatenate([vals[:cycles].astype(np.uint8),
+                           np.ones((cycles, len(fl)), np.uint8)], axis=1)
+    obs, regs = oracle.aig_sim(A, cycles, pins, bits, A["pos"], want_regs=True)
+    assert regs[n - 1].any() and not regs[n - 1].all()
+    obs2, regs2 = oracle.aig_sim(A, cycles - n, pins, bits[n:], A["pos"], want_regs=True,
+                                 dff_init=regs[n - 1])
+    assert np.array_equal(obs2, obs[n:]) and np.array_equal(regs2, regs[n:])
+    # starting from the wrong registers changes the outputs (the start
+    # state is used, not ignored)
+    obs3, _ = oracle.aig_sim(A, cycles - n, pins, bits[n:], A["pos"], want_regs=True)
+    assert not np.array_equal(obs3, obs[n:])
