@@ -1,23 +1,28 @@
 #!/usr/bin/env python3
 """Throughput of the MI355X Boolean-processor interpreter in simulated
-lane-cycles per second.
+lane-cycles per second: batched+learned against the plain GPU drop-in
+against the CPU path, bit-exact.
 
-Default workload (the plain GPU drop-in configuration): a seeded random AIG
-of 1M AND gates, depth 64, 64K DFFs, 512 inputs, 256 outputs, fan-in level
-distance ~ Geometric(0.3); 100K cycles of stimulus with every input toggling
-with probability 0.1.  One "step" is one launch of the interpreter over
---cycles-per-step cycles of that stimulus, continuing from the previous
-step's state, with the inputs already resident in HBM.
+Default workload (the headline): the 4M-gate netlist -- a seeded random AIG
+of 4M AND gates, depth 96, 256K DFFs, 8 single-port 1024x32 SRAMs, 512
+inputs, 256 outputs, fan-in level distance ~ Geometric(0.3) -- simulated as
+256 independent value lanes (per-lane seeds, cycle counts U[10K, 200K],
+toggle rates U[0.02, 0.5]) by the batched lane-sliced kernel with autotuned
+groups and blocks and the learned cost model's block assignment.  `value`
+is the whole lane job's lane-cycles over its wall time, with the lanes'
+stimulus generated on the device; one "step" is one graph replay of a
+resumable launch over an equal share of the job's group cycles.  Beside it:
+the plain drop-in on the same netlist, the reference CPU path (the
+oracle's literal restatement of sanity_check_cpu, one host core, a bounded
+cycle sample), and under config.forms the 1M-gate plain drop-in over 100K
+cycles and the 256 batched lanes of the 1M-gate netlist.  `roofline` is the
+batched kernel's: algorithmic HBM bytes per lane-cycle over its kernel time
+(HIP events on the launch stream), with the counter-measured traffic of the
+committed FETCH_SIZE / WRITE_SIZE profile of this workload beside it.
 
-The same run also measures the batched lane-sliced form (256 lanes of the
-same netlist, per-lane seeds, cycle counts U[10K, 200K] scaled by
---batched-scale, toggle rates U[0.02, 0.5]) and times the reference CPU
-path (the oracle's literal restatement of sanity_check_cpu) on a bounded
-cycle sample on one host core, with, labelled separately, the same
-restatement spread over the host's cores (OpenMP) and the direct AIG
-simulator.  Beside the HBM roofline it reports the LDS roofline (the
-interpreter's LDS bytes against the peak measured by an LDS probe kernel)
-and the grid-barrier time per major stage.
+Other workloads (--workload): plain (the 1M-gate drop-in line, with LDS
+roofline and barrier time), xtiming (4M gates with X, timing and SRAMs),
+mult16m (the 16M-gate structured netlist), cosim.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -45,17 +50,26 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=10,
+                    help="north: graph replays the batched job is split into; plain: "
+                         "timed launches of --cycles-per-step cycles")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cycles-per-step", type=int, default=10_000)
-    ap.add_argument("--workload", choices=["plain", "xtiming", "lanes4m", "mult16m",
-                                           "cosim"],
-                    default="plain",
-                    help="plain: the 1M-gate 2-state drop-in (the headline line); "
+    ap.add_argument("--plain-launches", type=int, default=2,
+                    help="north: launches of --cycles-per-step cycles of the plain "
+                         "drop-in on the 4M-gate netlist")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="north: skip the 1M-gate plain and batched forms")
+    ap.add_argument("--workload", choices=["north", "plain", "xtiming", "lanes4m",
+                                           "mult16m", "cosim"],
+                    default="north",
+                    help="north (the headline): batched+learned lanes vs the plain "
+                         "drop-in on the 4M-gate netlist, the 1M-gate forms beside it; "
+                         "plain: the 1M-gate 2-state drop-in; "
                          "xtiming: the 4M-gate netlist with 8 SRAMs, 10%% initial-X "
                          "DFFs, 20-200 ps gate delays and 50/20 ps setup/hold at 5 ns; "
-                         "lanes4m: batched+learned lanes vs the plain drop-in on that "
-                         "4M-gate netlist (value lanes); mult16m: the 16M-gate structured "
+                         "lanes4m: the headline without the 1M-gate forms; "
+                         "mult16m: the 16M-gate structured "
                          "netlist (64 16x16 multiplier tiles + 2M-gate control AIG, 1M "
                          "DFF), --lanes lanes of --cycles cycles, learned vs heuristic; "
                          "cosim: reactive co-simulation edges (SPI master + flash model, "
@@ -104,7 +118,9 @@ def parse():
                     help="first run an elementwise copy of MIB MiB three times "
                          "(the known-size kernel that scales the HBM counters)")
     a = ap.parse_args()
-    big = a.workload in ("xtiming", "lanes4m")
+    big = a.workload in ("xtiming", "lanes4m", "north")
+    if a.workload == "lanes4m":
+        a.workload, a.no_forms = "north", True
     if a.gates is None:
         a.gates = 4_000_000 if big else 1_000_000
     if a.depth is None:
@@ -235,14 +251,21 @@ def bench_plain(a, script, torch):
     return out
 
 
-def bench_batched(a, comp, torch, cycles=None, split=None):
+def bench_batched(a, comp, torch, cycles=None, split=None, steps=0, learned=None,
+                  widths=None):
     """Batched form on the same netlist: --lanes lanes with per-lane seeds,
     cycle counts U[10K, 200K] x --batched-scale and toggle rates
-    U[0.02, 0.5], in one persistent launch with the device lane queue,
-    captured with its reset as a hipGraph and timed as one graph replay.
+    U[0.02, 0.5], in one persistent launch with the device lane queue.
     Groups and blocks per group come from the autotuner unless given; the
     block assignment uses the learned partition costs (trained here on the
-    kernel's own partition timings) unless --no-learned."""
+    kernel's own partition timings) unless --no-learned (or learned=False).
+
+    steps = 0: the reset and the whole job captured as one hipGraph and
+    timed as one replay.  steps = K: the job is timed as K replays of a
+    graph of one resumable launch of ceil(1.02 * predicted group cycles / K)
+    group cycles each (after --warmup short launches and a reset), plus a
+    final launch to the end should the groups need more cycles than
+    predicted; the timed region covers the whole job either way."""
     from jacquard_amd import batched as B
     from jacquard_amd import costmodel as CM
     rng = np.random.default_rng(a.seed + 100)
@@ -251,13 +274,15 @@ def bench_batched(a, comp, torch, cycles=None, split=None):
     probs = rng.uniform(0.02, 0.5, a.lanes)
     if cycles is None:
         cycles = rng.integers(max(lo, 1), max(hi, 1) + 1, a.lanes)
+    if learned is None:
+        learned = not a.no_learned
     tune = None
     if a.batched_blocks:
         W = a.batched_blocks
         G = a.batched_groups or (a.lanes + 31) // 32
     else:
         t = time.time()
-        tune = B.autotune(comp, cycles, log=log)
+        tune = B.autotune(comp, cycles, log=log, **({"widths": widths} if widths else {}))
         W, G = tune["blocks"], tune["groups"]
         log(f"autotune: chose {G} groups x {W} blocks in {time.time() - t:.1f}s")
     width = None
@@ -266,37 +291,72 @@ def bench_batched(a, comp, torch, cycles=None, split=None):
         comp, width = B.tune_partition_width(
             comp, a.batched_split if split is None else split, W, G, log=log)
         log(f"width tune: {time.time() - t:.1f}s")
-    learned = None
+    lrn = None
     costs = None
-    if not a.no_learned:
+    if learned:
         costs, _model, info = CM.learn_costs(comp, W, G, log=log)
         ab = CM.compare_assignments(comp, W, G, {"heuristic": None, "learned": costs},
                                     repeats=3, cycles=32)
-        learned = dict(info, us_per_cycle=ab, gain=ab["heuristic"] / ab["learned"])
+        lrn = dict(info, us_per_cycle=ab, gain=ab["heuristic"] / ab["learned"])
         log(f"cost model: us/cycle heuristic {ab['heuristic']:.1f} learned "
             f"{ab['learned']:.1f}")
         if ab["learned"] > ab["heuristic"]:
             costs = None  # the heuristic stays the fallback; measured time judges
-            learned["used"] = "heuristic"
+            lrn["used"] = "heuristic"
         else:
-            learned["used"] = "learned"
+            lrn["used"] = "learned"
     script = comp.flatten_shared(W) if costs is None else comp.flatten(W, costs=costs)
     run = B.BatchedRun(script, seeds, probs, cycles, groups=G)
     script.validate(script.reg_io_state_size)
     run.prepare()
     torch.cuda.synchronize()
-    run.capture_graph()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record()
-    run.graph.replay()
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    stream = torch.cuda.current_stream()
     from jacquard_amd.sim import device_error
+    launch_ms, launches, drains, budget = [], 0, 0, 0
+    if steps <= 0:
+        run.capture_graph()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        run.graph.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        launch_ms.append(e0.elapsed_time(e1))
+        launches = 1
+    else:
+        pred = max(run.predicted_group_cycles())
+        budget = max(1, -(-int(1.02 * pred) // steps))
+        graph = run.capture_chunk_graph(budget)
+        for _ in range(a.warmup):
+            run.launch(budget=8)
+        torch.cuda.synchronize()
+        run.reset()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps + 1)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ev[k][0].record(stream)
+            graph.replay()
+            ev[k][1].record(stream)
+        torch.cuda.synchronize()
+        if not run.done():
+            ev[steps][0].record(stream)
+            run.launch()
+            ev[steps][1].record(stream)
+            drains = 1
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        launches = steps + drains
+        launch_ms = [s_.elapsed_time(e_) for s_, e_ in ev[:launches]]
+        log(f"batched: job of {steps} graph replays of {budget} group cycles "
+            f"(predicted {pred} group cycles){', plus a drain launch' if drains else ''}")
     if device_error():
         raise RuntimeError("device barrier timeout during the batched run")
+    if not run.done():
+        raise RuntimeError("batched job did not complete")
     lc = run.lane_cycles_total()
     log(f"batched: {a.lanes} lanes through {G} groups x {W} blocks, "
         f"{lc} lane-cycles in {wall:.2f}s")
@@ -315,9 +375,10 @@ def bench_batched(a, comp, torch, cycles=None, split=None):
         if not np.array_equal(outs[lane][:v], B.plain_observations(script, st, v, run.obs_lits)):
             raise AssertionError(f"batched GPU lane {lane} differs from the CPU oracle")
         checked += v
-    return dict(wall=wall, kernel_ms=e0.elapsed_time(e1), lane_cycles=lc,
+    return dict(wall=wall, kernel_ms=float(np.sum(launch_ms)), launch_ms=launch_ms,
+                launches=launches, drains=drains, budget=budget, lane_cycles=lc,
                 slot_cycles=run.slot_cycles_total(), groups=G, blocks=W,
-                verified_cycles=checked, script=script, learned=learned,
+                verified_cycles=checked, script=script, learned=lrn,
                 tune=None if tune is None else tune["trials"], width=width)
 
 
@@ -455,65 +516,162 @@ def bench_xtiming(a, torch):
                 cpu_s=cpu_s, verified_cycles=2 * V, counts=nl.counts())
 
 
-def bench_lanes4m(a, torch):
-    """Batched+learned lanes against the plain drop-in on the 4M-gate netlist
-    of the X/timing workload (same generator: 8 single-port 1024x32 SRAMs,
-    per-gate delays), both as 2-state value runs: the plain kernel runs
-    --steps x --cycles-per-step cycles after --warmup steps; the batched form
-    runs --lanes lanes of U[10K, 200K] x --batched-scale cycles."""
+def host_cpu():
+    """(CPU model, online cores, cores this process may use) of the host."""
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
+def netlist_4m(a):
     from jacquard_amd.netlist import Netlist
-    from jacquard_amd.script import Compiled
+    return Netlist.random(seed=a.seed, gates=a.gates, depth=a.depth, dffs=a.dffs,
+                          inputs=a.inputs, outputs=a.outputs, geometric_p=0.3,
+                          invert_prob=0.5, srams=8, sram_addr_bits=10, delay=(20, 200))
+
+
+def bench_north(a, torch):
+    """The headline: batched+learned lanes against the plain drop-in on the
+    4M-gate netlist of the X/timing workload (same generator: 8 single-port
+    1024x32 SRAMs; value lanes, 2-state), with the batched configuration's
+    lanes (--lanes lanes, cycles U[10K, 200K] x --batched-scale, toggle
+    U[0.02, 0.5]).  `value` is the batched job's lane-cycles over the wall
+    time of the whole job, run as --steps graph replays (after --warmup
+    short launches).  Beside it: the plain drop-in on the same netlist
+    (--plain-launches x --cycles-per-step cycles), and under config.forms
+    the 1M-gate plain drop-in (100K cycles) and the 256 batched lanes of the
+    1M-gate netlist (one graph replay), unless --no-forms."""
     import ctypes
     from jacquard_amd import _native as N
+    from jacquard_amd.script import Compiled
     t = time.time()
-    nl = Netlist.random(seed=a.seed, gates=a.gates, depth=a.depth, dffs=a.dffs,
-                        inputs=a.inputs, outputs=a.outputs, geometric_p=0.3,
-                        invert_prob=0.5, srams=8, sram_addr_bits=10, delay=(20, 200))
+    nl = netlist_4m(a)
     comp = Compiled(nl, level_split=a.split)
-    log(f"lanes4m: netlist {nl.counts()} compiled in {time.time() - t:.1f}s")
+    log(f"north: netlist {nl.counts()} compiled in {time.time() - t:.1f}s")
     res = ctypes.c_int(0)
     N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
     script = comp.flatten(min(max(comp.parts_per_stage()), res.value))
-    p = bench_plain(a, script, torch)
+    pa = argparse.Namespace(**vars(a))
+    pa.steps, pa.warmup = a.plain_launches, 1
+    p = bench_plain(pa, script, torch)
     plain = p["cycles"] / p["wall"]
-    log(f"lanes4m: plain {plain:,.0f} cycles/s")
-    if a.batched_split != a.split:
-        comp = Compiled(nl, level_split=a.batched_split)
-    b = bench_batched(a, comp, torch)
+    log(f"north: plain drop-in {plain:,.0f} cycles/s on the 4M-gate netlist")
+    comp_b = comp if a.batched_split == a.split else Compiled(nl, level_split=a.batched_split)
+    b = bench_batched(a, comp_b, torch, steps=a.steps, widths=(32, 48, 64))
     value = b["lane_cycles"] / b["wall"]
+    log(f"north: batched {value:,.0f} lane-cycles/s, {value / plain:.1f}x the plain drop-in")
+    bh = batched_hbm(b, BATCHED_4M_TRAFFIC_PROFILE)
+    forms = {}
+    if not a.no_forms:
+        forms = forms_1m(a, torch)
+    cpu = None
+    if not a.no_cpu:
+        model, online, usable = host_cpu()
+        c = cpu_baseline(a, script)
+        cpu = {"value": c["value"], "unit": "lane-cycles/s", "cores": 1, "kind": "port",
+               "sample": f"{c['cycles']} cycles of the 4M-gate netlist (2-state) through the "
+                         f"oracle's literal restatement of sanity_check_cpu (gcc -O2, one "
+                         f"thread) in {c['seconds']:.1f} s on {model}; {online} cores online, "
+                         f"{usable} usable by this process",
+               "cpu_model": model, "nproc": online, "usable_cores": usable}
+        log(f"cpu: {c['value']:.2f} lane-cycles/s over {c['cycles']} cycles")
     bs = b["script"]
-    bpc = bs.algorithmic_bytes_per_cycle()
-    # a group cycle moves the script once for its 32 lanes, plus 32 lanes'
-    # worth of state traffic: bytes per lane-cycle = W/32 + state part
-    per_lane = (4 * int(bs.blocks_start[-1])) / 32 + (bpc - 4 * int(bs.blocks_start[-1]))
-    achieved = per_lane * b["lane_cycles"] / (b["kernel_ms"] / 1e3) / 1e9
-    print(json.dumps({
+    line = {
         "metric": METRIC, "value": value, "unit": "lane-cycles/s", "n_gpus": 1,
-        "steps": 1, "warmup": 0, "ms_per_step": b["wall"] * 1e3,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": b["wall"] / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic",
         "config": {
             "workload": f"batched+learned lanes vs plain drop-in: seeded random AIG, "
-                        f"{a.gates} AND, depth {a.depth}, {a.dffs} DFF, 8 x 1024x32 SRAM; "
-                        f"{a.lanes} lanes, cycles U[{int(10_000 * a.batched_scale)}, "
+                        f"{a.gates} AND, depth {a.depth}, {a.dffs} DFF, 8 x 1024x32 SRAM, "
+                        f"{a.inputs} PI, {a.outputs} PO, geometric(0.3) fan-in; {a.lanes} "
+                        f"lanes, cycles U[{int(10_000 * a.batched_scale)}, "
                         f"{int(200_000 * a.batched_scale)}], toggle U[0.02, 0.5]",
             "batched": {"lanes": a.lanes, "groups": b["groups"],
                         "blocks_per_group": b["blocks"], "lane_cycles": b["lane_cycles"],
                         "slot_utilisation": b["lane_cycles"] / b["slot_cycles"],
                         "learned_cost_model": b["learned"], "autotune": b["tune"],
-                        "launch": "one hipGraph replay (reset + persistent kernel)",
+                        "partition_width": b["width"],
+                        "launch": f"{b['launches']} launches: {a.steps} replays of one "
+                                  f"hipGraph of a resumable persistent launch of "
+                                  f"{b['budget']} group cycles"
+                                  + (", then one launch to the end" if b["drains"] else ""),
+                        "kernel_s": b["kernel_ms"] / 1e3,
+                        "major_stages": bs.num_major_stages, "partitions": bs.num_parts,
+                        "script_bytes": int(bs.blocks_data.nbytes),
                         "verified_cycles": b["verified_cycles"]},
             "plain": {"lane_cycles_per_s": plain, "blocks": script.num_blocks,
+                      "cycles": p["cycles"], "major_stages": script.num_major_stages,
+                      "partitions": script.num_parts,
+                      "script_bytes": int(script.blocks_data.nbytes),
                       "verified_cycles": p.get("verified_cycles")},
             "vs_plain": value / plain,
-            "major_stages": script.num_major_stages, "partitions": script.num_parts,
-            "script_bytes": int(script.blocks_data.nbytes)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+            "forms": forms},
+        "roofline": {"bound": "hbm", "achieved": bh["achieved"], "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": bh["frac"], "traffic": bh["traffic"],
                      "kernel": "batched_kernel",
-                     "algorithmic_bytes_per_lane_cycle": per_lane},
-        "cpu_baseline": None,
-    }), flush=True)
+                     "algorithmic_bytes_per_lane_cycle": bh["algorithmic_bytes_per_lane_cycle"],
+                     "avg_launch_ms": b["kernel_ms"] / b["launches"],
+                     "measured_gbps": bh["traffic_gbps"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def forms_1m(a, torch):
+    """The 1M-gate workloads reported beside the headline: the plain
+    drop-in over 100K cycles (10 launches of 10K cycles; partition width
+    and blocks from sim.tune_plain) and the 256 batched lanes of the same
+    netlist (autotuned, tuned width, heuristic assignment, one graph
+    replay)."""
+    from jacquard_amd import sim
+    f = argparse.Namespace(**vars(a))
+    f.gates, f.depth, f.dffs = 1_000_000, 64, 65_536
+    f.steps, f.warmup, f.cycles_per_step = 10, 1, 10_000
+    nl, comp = build_design(f)
+    _comp, script, tune_table = sim.tune_plain(nl, f.split, log=log)
+    p = bench_plain(f, script, torch)
+    plain = p["cycles"] / p["wall"]
+    avg_launch_s = float(np.mean(p["launch_ms"])) / 1e3
+    bpc = script.algorithmic_bytes_per_cycle()
+    out = {"plain_1m": {
+        "workload": "plain GPU drop-in: seeded random AIG, 1000000 AND, depth 64, 65536 "
+                    "DFF, 512 PI, 256 PO, geometric(0.3) fan-in; 100000 cycles, toggle 0.1",
+        "lane_cycles_per_s": plain, "blocks": script.num_blocks,
+        "major_stages": script.num_major_stages, "partitions": script.num_parts,
+        "plain_partition_tune": tune_table, "verified_cycles": p.get("verified_cycles"),
+        "roofline": {"kernel": "scan_kernel", "achieved": bpc * f.cycles_per_step /
+                     avg_launch_s / 1e9, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "algorithmic_bytes_per_cycle": bpc, "avg_launch_ms": avg_launch_s * 1e3}}}
+    out["plain_1m"]["roofline"]["frac"] = out["plain_1m"]["roofline"]["achieved"] / HBM_PEAK_GBPS
+    log(f"forms: 1M plain {plain:,.0f} cycles/s")
+    from jacquard_amd.script import Compiled
+    comp_b = Compiled(nl, level_split=a.batched_split)
+    b = bench_batched(f, comp_b, torch, steps=0, learned=False)
+    bh = batched_hbm(b)
+    out["batched_1m"] = {
+        "workload": "256 batched lanes of the 1M-gate netlist, cycles U[10K, 200K], "
+                    "toggle U[0.02, 0.5]; one hipGraph replay of the reset and the "
+                    "persistent launch",
+        "lane_cycles_per_s": b["lane_cycles"] / b["wall"], "lanes": f.lanes,
+        "groups": b["groups"], "blocks_per_group": b["blocks"],
+        "lane_cycles": b["lane_cycles"], "partition_width": b["width"],
+        "slot_utilisation": b["lane_cycles"] / b["slot_cycles"],
+        "vs_plain_1m": b["lane_cycles"] / b["wall"] / plain, "hbm": bh,
+        "verified_cycles": b["verified_cycles"]}
+    log(f"forms: 1M batched {b['lane_cycles'] / b['wall']:,.0f} lane-cycles/s")
+    return out
 
 
 def bench_mult16m(a, torch):
@@ -842,9 +1000,11 @@ def barrier_us_per_stage(script, torch, cycles=2000):
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "scan_kernel_1M_gates_hbm_traffic.json")
 BATCHED_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "batched_kernel_1M_gates_hbm_traffic.json")
+BATCHED_4M_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
+                                          "batched_kernel_4M_gates_hbm_traffic.json")
 
 
-def batched_hbm(b):
+def batched_hbm(b, profile=None):
     """HBM figures of the batched launch: algorithmic bytes per lane-cycle
     (a group cycle streams the script once for its 32 lanes, plus every
     lane's state traffic) over the kernel time, and the measured bytes per
@@ -857,10 +1017,16 @@ def batched_hbm(b):
     out = {"algorithmic_bytes_per_lane_cycle": per_lane, "achieved": per_lane * rate / 1e9,
            "peak": HBM_PEAK_GBPS, "unit": "GB/s", "traffic_gbps": None}
     out["frac"] = out["achieved"] / HBM_PEAK_GBPS
-    if os.path.exists(BATCHED_TRAFFIC_PROFILE):
-        with open(BATCHED_TRAFFIC_PROFILE) as f:
+    out["traffic"] = None
+    profile = profile or BATCHED_TRAFFIC_PROFILE
+    if os.path.exists(profile):
+        with open(profile) as f:
             prof = json.load(f)
-        out["traffic_gbps"] = prof["hbm_bytes_per_lane_cycle"] * rate / 1e9
+        if prof.get("script_bytes") in (None, int(bs.blocks_data.nbytes)):
+            out["traffic_gbps"] = prof["hbm_bytes_per_lane_cycle"] * rate / 1e9
+            # HBM bytes per launch at this run's lane-cycles per launch
+            out["traffic"] = prof["hbm_bytes_per_lane_cycle"] * b["lane_cycles"] / max(
+                b.get("launches", 1), 1)
     return out
 
 
@@ -891,8 +1057,8 @@ def main():
         sys.exit(1)
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
-    if a.workload == "lanes4m":
-        return bench_lanes4m(a, torch)
+    if a.workload == "north":
+        return bench_north(a, torch)
     if a.workload == "mult16m":
         return bench_mult16m(a, torch)
     if a.workload == "cosim":

@@ -233,6 +233,43 @@ class BatchedRun:
         self.graph = g
         return g
 
+    def capture_chunk_graph(self, budget):
+        """A hipGraph of one resumable launch of `budget` group cycles (no
+        reset): every replay continues the job where the previous replay
+        or launch left it, so the job runs as a sequence of replays.
+        Returns the graph."""
+        import torch
+        if budget < 1:
+            raise ValueError("a chunk needs a budget of at least one group cycle")
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.launch(stream=s, budget=budget)
+        torch.cuda.current_stream().wait_stream(s)
+        return g
+
+    def predicted_group_cycles(self):
+        """Group cycles each group runs when the groups advance in step:
+        the device queue replayed on the host (lanes longest first, a freed
+        slot takes the next lane at the cycle its lane ended).  On the
+        device, groups that drift apart can take lanes in another order, so
+        this is a close estimate, not a bound."""
+        import heapq
+        q = [int(c) for c in self.q_cycles.tolist()]
+        heap = []
+        for k in range(self.first):
+            heapq.heappush(heap, (q[k], k % self.G, k // self.G))
+        nxt, gend = self.first, [0] * self.G
+        while heap:
+            end, g, l = heapq.heappop(heap)
+            gend[g] = max(gend[g], end)
+            if nxt < len(q):
+                heapq.heappush(heap, (end + q[nxt], g, l))
+                nxt += 1
+        return gend
+
     def _args(self):
         s = self.script
         d = self.d
