@@ -109,6 +109,9 @@ def parse():
     ap.add_argument("--no-width-tune", action="store_true",
                     help="batched form with the compiler's default partition width "
                          "instead of the measured choice (batched.tune_partition_width)")
+    ap.add_argument("--tune-splits", default="2,3,4",
+                    help="level splits the batched form's tiling tune also tries "
+                         "(comma-separated; empty: only --batched-split)")
     ap.add_argument("--no-learned", action="store_true",
                     help="batched form with the heuristic block assignment only")
     ap.add_argument("--no-cpu", action="store_true")
@@ -289,22 +292,32 @@ def bench_batched(a, comp, torch, cycles=None, split=None, steps=0, learned=None
     if not a.no_width_tune:
         t = time.time()
         comp, width = B.tune_partition_width(
-            comp, a.batched_split if split is None else split, W, G, log=log)
+            comp, a.batched_split if split is None else split, W, G, log=log,
+            splits=tuple(int(x) for x in a.tune_splits.split(",") if x))
         log(f"width tune: {time.time() - t:.1f}s")
     lrn = None
     costs = None
     if learned:
-        costs, _model, info = CM.learn_costs(comp, W, G, log=log)
-        ab = CM.compare_assignments(comp, W, G, {"heuristic": None, "learned": costs},
-                                    repeats=3, cycles=32)
-        lrn = dict(info, us_per_cycle=ab, gain=ab["heuristic"] / ab["learned"])
-        log(f"cost model: us/cycle heuristic {ab['heuristic']:.1f} learned "
-            f"{ab['learned']:.1f}")
-        if ab["learned"] > ab["heuristic"]:
+        # the learned model sets the block assignment and decides the
+        # partition merges; measured kernel time picks among the heuristic,
+        # the learned assignment and the learned merge + assignment
+        costs, model, info = CM.learn_costs(comp, W, G, log=log)
+        merged, mcosts, minfo = CM.learned_merge(comp, model, W, groups=G, log=log)
+        variants = {"heuristic": None, "learned": costs}
+        if merged is not comp:
+            variants["learned+merge"] = (merged, mcosts)
+        spread = {}
+        ab = CM.compare_assignments(comp, W, G, variants, repeats=3, cycles=32,
+                                    spread=spread)
+        best = min(ab, key=ab.get)
+        lrn = dict(info, merge=minfo, us_per_cycle=ab, spread_us=spread,
+                   gain=ab["heuristic"] / ab[best], used=best)
+        log("cost model: us/cycle " + ", ".join(f"{k} {v:.1f}" for k, v in ab.items())
+            + f"; using {best}")
+        if best == "heuristic":
             costs = None  # the heuristic stays the fallback; measured time judges
-            lrn["used"] = "heuristic"
-        else:
-            lrn["used"] = "learned"
+        elif best == "learned+merge":
+            comp, costs = merged, mcosts
     script = comp.flatten_shared(W) if costs is None else comp.flatten(W, costs=costs)
     run = B.BatchedRun(script, seeds, probs, cycles, groups=G)
     script.validate(script.reg_io_state_size)

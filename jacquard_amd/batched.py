@@ -451,36 +451,57 @@ def autotune(compiled, lane_cycles, widths=(16, 24, 32, 48, 64), trial_cycles=24
 
 
 def tune_partition_width(compiled, level_split, blocks, groups,
-                         caps=(4000, 1800, 1300, 1100, 900), trial_cycles=48, log=None):
-    """Partition width for the batched form at a chosen (blocks, groups).
+                         caps=(4000, 1800, 1300, 1100, 900), trial_cycles=48, log=None,
+                         splits=()):
+    """Partition tiling for the batched form at a chosen (blocks, groups).
 
-    Capping the nodes per logic level of a partition (``max_level_nodes``)
-    gives more, narrower partitions with fewer boomerang stages each; once a
-    major stage holds more partitions than the group has blocks they
-    serialise.  The given compilation and every cap whose widest major stage
-    is at most 1.5x the block count are flattened for `blocks` and timed over
-    `trial_cycles` cycles of `groups` full groups; the fastest wins.
-    Returns (Compiled, table)."""
+    A shorter level split gives more major stages (more group barriers per
+    cycle) with fewer, deeper partitions in each; capping the nodes per
+    logic level of a partition (``max_level_nodes``) gives more, narrower
+    partitions with fewer boomerang stages each.  Once a major stage holds
+    more partitions than the group has blocks they serialise.  The given
+    compilation, then every other level split in `splits`, then every cap
+    at the best split, whose widest major stage is at most 1.5x the block
+    count, are flattened for `blocks` and timed over `trial_cycles` cycles
+    of `groups` full groups; the fastest wins.  Returns (Compiled, table)."""
     from .costmodel import batched_us_per_cycle
     from .script import Compiled
     table = []
     us = batched_us_per_cycle(compiled.flatten_shared(blocks), groups, cycles=trial_cycles)
-    table.append({"max_level_nodes": None, "widest_stage": max(compiled.parts_per_stage()),
+    table.append({"level_split": level_split, "max_level_nodes": None,
+                  "widest_stage": max(compiled.parts_per_stage()),
                   "tried": True, "us_per_cycle": us})
     best = (compiled, us)
+    best_split = level_split
     if log:
-        log(f"width tune: as compiled: {us:.1f} us/cycle")
-    for cap in caps:
-        comp = Compiled(compiled.netlist, level_split=level_split, max_level_nodes=cap)
+        log(f"width tune: as compiled (level split {level_split}): {us:.1f} us/cycle")
+    for sp in splits:
+        if sp == level_split:
+            continue
+        comp = Compiled(compiled.netlist, level_split=sp)
         widest = max(comp.parts_per_stage())
-        row = {"max_level_nodes": cap, "widest_stage": widest}
+        row = {"level_split": sp, "max_level_nodes": None, "widest_stage": widest}
         if widest > 1.5 * blocks:
             table.append(dict(row, tried=False))
             continue
         us = batched_us_per_cycle(comp.flatten_shared(blocks), groups, cycles=trial_cycles)
         table.append(dict(row, tried=True, us_per_cycle=us))
         if log:
-            log(f"width tune: max_level_nodes {cap}: widest stage {widest}, {us:.1f} us/cycle")
+            log(f"width tune: level split {sp}: widest stage {widest}, {us:.1f} us/cycle")
+        if us < best[1]:
+            best, best_split = (comp, us), sp
+    for cap in caps:
+        comp = Compiled(compiled.netlist, level_split=best_split, max_level_nodes=cap)
+        widest = max(comp.parts_per_stage())
+        row = {"level_split": best_split, "max_level_nodes": cap, "widest_stage": widest}
+        if widest > 1.5 * blocks:
+            table.append(dict(row, tried=False))
+            continue
+        us = batched_us_per_cycle(comp.flatten_shared(blocks), groups, cycles=trial_cycles)
+        table.append(dict(row, tried=True, us_per_cycle=us))
+        if log:
+            log(f"width tune: level split {best_split}, max_level_nodes {cap}: widest "
+                f"stage {widest}, {us:.1f} us/cycle")
         if us < best[1]:
             best = (comp, us)
     return best[0], table

@@ -205,17 +205,111 @@ def batched_us_per_cycle(script, groups, cycles=48, seed=1):
     return e0.elapsed_time(e1) * 1e3 / cycles
 
 
-def compare_assignments(compiled, blocks, groups, variants, repeats=3, cycles=48):
-    """Measured us/cycle of the batched kernel for each named cost variant
-    ({name: costs or None}); the variants' launches are interleaved
-    `repeats` times and the median is returned per name."""
-    scripts = {k: (compiled.flatten_shared(blocks) if c is None
-                   else compiled.flatten(blocks, costs=c)) for k, c in variants.items()}
+def compare_assignments(compiled, blocks, groups, variants, repeats=3, cycles=48,
+                        spread=None):
+    """Measured us/cycle of the batched kernel for each named variant
+    ({name: costs or None, or (compiled design, costs or None)}); the
+    variants' launches are interleaved `repeats` times and the median is
+    returned per name.  With a dict as `spread`, each name's max - min over
+    the repeats is stored in it."""
+    def script(v):
+        c, costs = v if isinstance(v, tuple) else (compiled, v)
+        return c.flatten_shared(blocks) if costs is None else c.flatten(blocks, costs=costs)
+    scripts = {k: script(v) for k, v in variants.items()}
     out = {k: [] for k in variants}
     for r in range(repeats):
         for k in variants:
             out[k].append(batched_us_per_cycle(scripts[k], groups, cycles, seed=r + 1))
+    if spread is not None:
+        spread.update({k: float(np.max(v) - np.min(v)) for k, v in out.items()})
     return {k: float(np.median(v)) for k, v in out.items()}
+
+
+def lpt_makespan(costs, blocks):
+    """Largest block total when partitions of these costs go, largest first,
+    to the least-loaded of `blocks` blocks (the flatten's assignment)."""
+    import heapq
+    heap = [0.0] * blocks
+    for x in sorted((float(c) for c in costs), reverse=True):
+        heapq.heapreplace(heap, heap[0] + x)
+    return max(heap) if heap else 0.0
+
+
+def merge_candidates(pred, blocks):
+    """Pairs (major, i, j) proposed for merging: in every major stage with
+    n > blocks partitions, the 2k of smallest predicted latency, k =
+    min(n - blocks, n // 2), paired smallest with largest among them so that
+    the merged partitions' predicted latencies stay even (with k = n -
+    blocks the stage then fits one partition per block)."""
+    pairs = []
+    for m, c in enumerate(pred):
+        k = min(len(c) - blocks, len(c) // 2)
+        if k <= 0:
+            continue
+        small = np.argsort(np.asarray(c), kind="stable")[:2 * k]
+        pairs += [(m, int(small[a]), int(small[2 * k - 1 - a])) for a in range(k)]
+    return pairs
+
+
+def learned_merge(compiled, model, blocks, groups=None, sample_cycles=16, epochs=100,
+                  log=None):
+    """The partition-merge step driven by the learned latency model (the
+    role of the merge acceptance of src/pe.rs:1033-1046).  In each major
+    stage with more partitions than blocks, the candidates of
+    merge_candidates are mapped as merged partitions; the model predicts
+    the merged partitions' latencies from their script sections, and a
+    stage keeps its merges when its predicted makespan (largest block total
+    after the longest-first assignment of predicted costs) drops.
+
+    Merged partitions are larger than those the model was trained on, so
+    with `groups` given (and the model's training set kept by learn_costs)
+    the trial design is profiled first (sample_cycles cycles of `groups`
+    lane groups) and the model refitted on the old and new samples before
+    it predicts.  Returns (compiled design with the kept merges, its learned
+    costs, info); with nothing kept, the design is `compiled` itself."""
+    s0 = compiled.flatten_shared(blocks)
+    pred0 = stage_costs(compiled, s0, model.predict(s0.part_features()))
+    before = [lpt_makespan(c, blocks) for c in pred0]
+    info = dict(candidates=0, merged=0, stages_kept=0, refit_samples=0,
+                predicted_ticks_before=float(sum(before)))
+    pairs = merge_candidates(pred0, blocks)
+    info["candidates"] = len(pairs)
+    if not pairs:
+        info["predicted_ticks_after"] = info["predicted_ticks_before"]
+        return compiled, pred0, info
+    trial, ok = compiled.merged(pairs)
+    s1 = trial.flatten_shared(blocks)
+    if groups and ok.any() and getattr(model, "train_x", None) is not None:
+        _s, ticks = profile_batched(trial, blocks, groups, sample_cycles + 2, seed=7)
+        t = ticks[2:]
+        f1 = s1.part_features()
+        X = np.concatenate([model.train_x, np.repeat(f1[None], t.shape[0], 0).reshape(-1, f1.shape[1])])
+        y = np.concatenate([model.train_y, t.reshape(-1)])
+        model.fit(X.astype(np.float32), y, epochs=epochs, lr=0.05, batch=4096)
+        model.train_x, model.train_y = X.astype(np.float32), y
+        info["refit_samples"] = int(t.size)
+        pred0 = stage_costs(compiled, s0, model.predict(s0.part_features()))
+        before = [lpt_makespan(c, blocks) for c in pred0]
+        info["predicted_ticks_before"] = float(sum(before))
+    pred1 = stage_costs(trial, s1, model.predict(s1.part_features()))
+    after = [lpt_makespan(c, blocks) for c in pred1]
+    keep = {m for m in range(len(pred0)) if after[m] < before[m]}
+    kept = [p for p, o in zip(pairs, ok) if o and p[0] in keep]
+    info.update(merged=int(ok.sum()), stages_kept=len(keep), pairs_kept=len(kept),
+                predicted_ticks_after=float(sum(after[m] if m in keep else before[m]
+                                                for m in range(len(pred0)))))
+    if log:
+        log(f"learned merge: {len(pairs)} candidate pairs, {int(ok.sum())} fit one "
+            f"partition; {len(keep)} of {len(pred0)} major stages keep theirs; predicted "
+            f"ticks per cycle {info['predicted_ticks_before']:.0f} -> "
+            f"{info['predicted_ticks_after']:.0f}")
+    if not kept:
+        return compiled, pred0, info
+    if len(kept) == int(ok.sum()):
+        return trial, pred1, info
+    final, _ = compiled.merged(kept)
+    sf = final.flatten_shared(blocks)
+    return final, stage_costs(final, sf, model.predict(sf.part_features())), info
 
 
 def learn_costs(compiled, blocks, groups, sample_cycles=24, epochs=150, log=None):
@@ -231,6 +325,7 @@ def learn_costs(compiled, blocks, groups, sample_cycles=24, epochs=150, log=None
     t = time.time()
     model = MLP(seed=1)
     hist = model.fit(X, y, epochs=epochs, lr=0.05, batch=4096)
+    model.train_x, model.train_y = X, y  # kept for refits on new samples
     t_fit = time.time() - t
     s0 = compiled.flatten_shared(blocks)
     f0 = s0.part_features()

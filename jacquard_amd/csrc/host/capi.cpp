@@ -201,6 +201,26 @@ void *jqh_compile(void *aig, uint32_t level_split, uint32_t max_inputs,
 
 void jqh_compiled_free(void *h) { delete static_cast<Compiled *>(h); }
 
+// A new compiled design with the pairs (major, i, j) of `compiled` merged
+// (merge_partitions); merged_out[k] = 1 for each pair that merged.
+void *jqh_compiled_merge(void *aig, void *compiled, const uint32_t *pairs,
+                         uint64_t n_pairs, uint8_t *merged_out) {
+  auto *g = static_cast<Aig *>(aig);
+  auto *c = static_cast<Compiled *>(compiled);
+  auto *o = new Compiled();
+  std::vector<uint8_t> merged;
+  if (guard([&] { merge_partitions(*g, *c, pairs, n_pairs, *o, merged); })) {
+    delete o;
+    return nullptr;
+  }
+  for (size_t k = 0; k < merged.size(); ++k) merged_out[k] = merged[k];
+  return o;
+}
+
+uint32_t jqh_compiled_part_endpoints(void *h, uint32_t m, uint32_t p) {
+  return (uint32_t)static_cast<Compiled *>(h)->stages[m].parts[p].endpoints.size();
+}
+
 uint32_t jqh_compiled_num_major(void *h) {
   return (uint32_t)static_cast<Compiled *>(h)->stages.size();
 }

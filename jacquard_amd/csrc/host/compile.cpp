@@ -566,6 +566,99 @@ void endpoint_outputs(const Aig &g, const MajorStage &ms, size_t e,
   }
 }
 
+// Per-thread mapping context (stamp arrays over all pins).
+struct MapCtx {
+  Scratch sc;
+  std::vector<uint32_t> mark, seen, stack;
+  uint32_t gstamp = 1, sstamp = 1;
+  explicit MapCtx(size_t n) : sc(n), mark(n, 0), seen(n, 0) {}
+};
+
+// Map the endpoints `eps` of major stage `ms` (endpoint bookkeeping in
+// `infos`; `avail_pin` says which pins earlier stages provide) as one
+// partition.  Returns false when they do not fit one: more than 8192 input
+// bits, too many SRAM / duplicate writeout words, or a cone the mappers
+// cannot place.
+bool map_group(const Aig &g, const CompileOptions &opt, const MajorStage &ms,
+               const std::vector<EndpointInfo> &infos,
+               const std::function<bool(uint32_t)> &avail_pin,
+               const std::vector<uint32_t> &eps, MapCtx &cx, Partition &part) {
+  Scratch &sc = cx.sc;
+  std::vector<uint32_t> &mark = cx.mark, &seen = cx.seen, &stack = cx.stack;
+  uint32_t &gstamp = cx.gstamp, &sstamp = cx.sstamp;
+  MapInput mi;
+  ++sstamp;
+  uint32_t nsram = 0;
+  for (auto e : eps) {
+    nsram += infos[e].nsram;
+    for (auto p : infos[e].outs) {
+      if (p == 0) continue;
+      if (seen[p] != sstamp) {
+        seen[p] = sstamp;
+        mi.outputs.push_back(p);
+      }
+    }
+  }
+  // cone closure
+  ++gstamp;
+  for (auto p : mi.outputs) stack.push_back(p);
+  while (!stack.empty()) {
+    uint32_t p = stack.back();
+    stack.pop_back();
+    if (p == 0 || mark[p] == gstamp) continue;
+    mark[p] = gstamp;
+    if (avail_pin(p)) {
+      mi.inputs.push_back(p);
+      continue;
+    }
+    mi.nodes.push_back(p);
+    stack.push_back(g.a[p] >> 1);
+    stack.push_back(g.b[p] >> 1);
+  }
+  std::sort(mi.nodes.begin(), mi.nodes.end());
+  std::sort(mi.inputs.begin(), mi.inputs.end());
+  // duplicate words: distinct activations of one pin beyond the first
+  std::map<uint32_t, std::vector<uint32_t>> act;
+  for (auto e : eps) {
+    if (e < ms.staged_out.size()) {
+      auto &v = act[ms.staged_out[e]];
+      if (std::find(v.begin(), v.end(), 2u) == v.end()) v.push_back(2u);
+      continue;
+    }
+    size_t oe = ms.orig_endpoints[e - ms.staged_out.size()];
+    uint32_t pin = 0, key = 0;
+    if (oe < g.pos.size()) {
+      if (g.pos[oe] <= 1) continue;
+      pin = g.pos[oe] >> 1;
+      key = 2 | (g.pos[oe] & 1);
+    } else if (oe < g.pos.size() + g.dffs.size()) {
+      const Dff &d = g.dffs[oe - g.pos.size()];
+      if (d.d_iv <= 1) continue;
+      pin = d.d_iv >> 1;
+      key = d.en_iv << 1 | (d.d_iv & 1);
+    } else {
+      continue;
+    }
+    auto &v = act[pin];
+    if (std::find(v.begin(), v.end(), key) == v.end()) v.push_back(key);
+  }
+  uint32_t ndup = 0;
+  for (auto &kv : act) ndup += (uint32_t)kv.second.size() - 1;
+  uint32_t dup_words = (ndup + 31) / 32;
+  if (mi.inputs.size() > kWidth || nsram * 4 + dup_words > 256 || nsram + dup_words >= 256)
+    return false;
+  std::vector<BoomStage> stages;
+  // multi-level mapping first, the level-1 mapper as the fallback
+  bool m = opt.multilevel &&
+           map_partition_ml(g, mi, sc, stages, 256 - nsram - dup_words, opt.multilevel);
+  if (!m) m = map_partition(g, mi, sc, stages, 256 - nsram - dup_words);
+  if (!m) return false;
+  part.endpoints = eps;
+  part.stages = std::move(stages);
+  part.num_nodes = mi.nodes.size();
+  return true;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -605,6 +698,8 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
       last_use[p] = std::max(last_use[p], ep_stage[e]);
     });
   out.stages.assign(nmajor, MajorStage());
+  out.level_split = split;
+  out.opt = opt;
   for (uint32_t p = 1; p <= g.num_pins; ++p) {
     int s = stage_of(p);
     if (s >= 0 && last_use[p] > s) out.stages[s].staged_out.push_back(p);
@@ -618,12 +713,6 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
   size_t total_parts = 0, total_bstages = 0, total_nodes = 0;
 
   // per-thread mapping contexts (stamp arrays over all pins), made on first use
-  struct MapCtx {
-    Scratch sc;
-    std::vector<uint32_t> mark, seen, stack;
-    uint32_t gstamp = 1, sstamp = 1;
-    explicit MapCtx(size_t n) : sc(n), mark(n, 0), seen(n, 0) {}
-  };
   std::vector<std::unique_ptr<MapCtx>> ctxs;
 
   for (uint32_t m = 0; m < nmajor; ++m) {
@@ -724,96 +813,20 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
     // map every group (in parallel, each thread with its own stamp arrays);
     // split a group in halves when it does not fit
     std::vector<std::vector<Partition>> built(group_eps.size());
+    const std::function<bool(uint32_t)> avail = avail_pin;
     std::function<void(MapCtx &, std::vector<uint32_t>, std::vector<Partition> &)> build =
         [&](MapCtx &cx, std::vector<uint32_t> eps, std::vector<Partition> &res) {
-      Scratch &sc = cx.sc;
-      std::vector<uint32_t> &mark = cx.mark, &seen = cx.seen, &stack = cx.stack;
-      uint32_t &gstamp = cx.gstamp, &sstamp = cx.sstamp;
-      MapInput mi;
-      ++sstamp;
-      uint32_t nsram = 0;
-      for (auto e : eps) {
-        nsram += infos[e].nsram;
-        for (auto p : infos[e].outs) {
-          if (p == 0) continue;
-          if (seen[p] != sstamp) {
-            seen[p] = sstamp;
-            mi.outputs.push_back(p);
-            stack.push_back(p);
-          }
-        }
-      }
-      // cone closure
-      ++gstamp;
-      std::vector<uint32_t> outs_copy = mi.outputs;
-      for (auto p : outs_copy) stack.push_back(p);
-      while (!stack.empty()) {
-        uint32_t p = stack.back();
-        stack.pop_back();
-        if (p == 0 || mark[p] == gstamp) continue;
-        mark[p] = gstamp;
-        if (avail_pin(p)) {
-          mi.inputs.push_back(p);
-          continue;
-        }
-        mi.nodes.push_back(p);
-        stack.push_back(g.a[p] >> 1);
-        stack.push_back(g.b[p] >> 1);
-      }
-      std::sort(mi.nodes.begin(), mi.nodes.end());
-      std::sort(mi.inputs.begin(), mi.inputs.end());
-      // duplicate words: distinct activations of one pin beyond the first
-      std::map<uint32_t, std::vector<uint32_t>> act;
-      for (auto e : eps) {
-        if (e < ms.staged_out.size()) {
-          auto &v = act[ms.staged_out[e]];
-          if (std::find(v.begin(), v.end(), 2u) == v.end()) v.push_back(2u);
-          continue;
-        }
-        size_t oe = ms.orig_endpoints[e - ms.staged_out.size()];
-        uint32_t pin = 0, key = 0;
-        if (oe < g.pos.size()) {
-          if (g.pos[oe] <= 1) continue;
-          pin = g.pos[oe] >> 1;
-          key = 2 | (g.pos[oe] & 1);
-        } else if (oe < g.pos.size() + g.dffs.size()) {
-          const Dff &d = g.dffs[oe - g.pos.size()];
-          if (d.d_iv <= 1) continue;
-          pin = d.d_iv >> 1;
-          key = d.en_iv << 1 | (d.d_iv & 1);
-        } else {
-          continue;
-        }
-        auto &v = act[pin];
-        if (std::find(v.begin(), v.end(), key) == v.end()) v.push_back(key);
-      }
-      uint32_t ndup = 0;
-      for (auto &kv : act) ndup += (uint32_t)kv.second.size() - 1;
-      uint32_t dup_words = (ndup + 31) / 32;
-      bool ok = mi.inputs.size() <= kWidth && nsram * 4 + dup_words <= 256 &&
-                nsram + dup_words < 256;
-      std::vector<BoomStage> stages;
-      if (ok) {
-        // multi-level mapping first, the level-1 mapper as the fallback
-        bool m = opt.multilevel &&
-                 map_partition_ml(g, mi, sc, stages, 256 - nsram - dup_words, opt.multilevel);
-        if (!m) m = map_partition(g, mi, sc, stages, 256 - nsram - dup_words);
-        ok = m;
-      }
-      if (!ok) {
-        if (eps.size() <= 1)
-          throw std::runtime_error("compile: a single endpoint does not fit a partition");
-        std::vector<uint32_t> a(eps.begin(), eps.begin() + eps.size() / 2);
-        std::vector<uint32_t> b(eps.begin() + eps.size() / 2, eps.end());
-        build(cx, a, res);
-        build(cx, b, res);
+      Partition part;
+      if (map_group(g, opt, ms, infos, avail, eps, cx, part)) {
+        res.push_back(std::move(part));
         return;
       }
-      Partition part;
-      part.endpoints = eps;
-      part.stages = std::move(stages);
-      part.num_nodes = mi.nodes.size();
-      res.push_back(std::move(part));
+      if (eps.size() <= 1)
+        throw std::runtime_error("compile: a single endpoint does not fit a partition");
+      std::vector<uint32_t> a(eps.begin(), eps.begin() + eps.size() / 2);
+      std::vector<uint32_t> b(eps.begin() + eps.size() / 2, eps.end());
+      build(cx, a, res);
+      build(cx, b, res);
     };
     std::string par_err;
 #pragma omp parallel
@@ -867,6 +880,103 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
      << " boomerang stages, " << total_nodes << " mapped AND evaluations ("
      << g.num_and() << " unique)\n";
   if (log) *log = lg.str();
+}
+
+void merge_partitions(const Aig &g, const Compiled &in, const uint32_t *pairs,
+                      size_t n_pairs, Compiled &out, std::vector<uint8_t> &merged) {
+  const uint32_t nmajor = (uint32_t)in.stages.size();
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> by_stage(nmajor);
+  std::vector<std::vector<size_t>> pair_id(nmajor);
+  for (size_t k = 0; k < n_pairs; ++k) {
+    uint32_t m = pairs[3 * k], i = pairs[3 * k + 1], j = pairs[3 * k + 2];
+    if (m >= nmajor || i >= in.stages[m].parts.size() || j >= in.stages[m].parts.size() ||
+        i == j)
+      throw std::runtime_error("merge: pair out of range");
+    by_stage[m].push_back({i, j});
+    pair_id[m].push_back(k);
+  }
+  Levels L = levelize(g);
+  const uint32_t split = in.level_split ? in.level_split : 1;
+  auto stage_of = [&](uint32_t p) -> int {
+    if (g.drv[p] != kAnd) return -1;
+    return (int)std::min<uint32_t>((L.lvl[p] - 1) / split, nmajor - 1);
+  };
+  out.stages.clear();
+  out.stages.reserve(nmajor);
+  out.part_x = in.part_x;
+  out.level_split = in.level_split;
+  out.opt = in.opt;
+  merged.assign(n_pairs, 0);
+  std::vector<std::unique_ptr<MapCtx>> ctxs;
+  for (uint32_t m = 0; m < nmajor; ++m) {
+    const MajorStage &ms = in.stages[m];
+    MajorStage o;
+    o.staged_out = ms.staged_out;
+    o.orig_endpoints = ms.orig_endpoints;
+    const auto &pl = by_stage[m];
+    std::vector<uint8_t> used(ms.parts.size(), 0);
+    for (auto &pr : pl) {
+      if (used[pr.first] || used[pr.second])
+        throw std::runtime_error("merge: a partition appears in two pairs of one stage");
+      used[pr.first] = used[pr.second] = 1;
+    }
+    std::vector<EndpointInfo> infos;
+    if (!pl.empty()) {
+      infos.resize(ms.num_endpoints());
+      for (size_t e = 0; e < infos.size(); ++e) endpoint_outputs(g, ms, e, infos[e]);
+    }
+    const std::function<bool(uint32_t)> avail = [&](uint32_t p) {
+      return stage_of(p) < (int)m;
+    };
+    std::vector<Partition> res(pl.size());
+    std::vector<uint8_t> ok(pl.size(), 0);
+    std::string par_err;
+#pragma omp parallel
+    {
+      int tid = 0;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+#pragma omp single
+      {
+        int nt = 1;
+#ifdef _OPENMP
+        nt = omp_get_num_threads();
+#endif
+        if ((int)ctxs.size() < nt) ctxs.resize(nt);
+      }
+#pragma omp for schedule(dynamic, 1)
+      for (int64_t k = 0; k < (int64_t)pl.size(); ++k) {
+        try {
+          if (!ctxs[tid]) ctxs[tid] = std::make_unique<MapCtx>(g.num_pins + 1);
+          std::vector<uint32_t> eps = ms.parts[pl[k].first].endpoints;
+          const auto &b = ms.parts[pl[k].second].endpoints;
+          eps.insert(eps.end(), b.begin(), b.end());
+          std::sort(eps.begin(), eps.end());
+          ok[k] = map_group(g, in.opt, ms, infos, avail, eps, *ctxs[tid], res[k]) ? 1 : 0;
+        } catch (const std::exception &e) {
+#pragma omp critical(jq_merge_err)
+          if (par_err.empty()) par_err = e.what();
+        }
+      }
+    }
+    if (!par_err.empty()) throw std::runtime_error(par_err);
+    std::vector<uint8_t> drop(ms.parts.size(), 0);
+    for (size_t k = 0; k < pl.size(); ++k)
+      if (ok[k]) {
+        drop[pl[k].first] = drop[pl[k].second] = 1;
+        merged[pair_id[m][k]] = 1;
+      }
+    for (size_t i = 0; i < ms.parts.size(); ++i)
+      if (!drop[i]) o.parts.push_back(ms.parts[i]);
+    for (size_t k = 0; k < pl.size(); ++k)
+      if (ok[k]) o.parts.push_back(std::move(res[k]));
+    std::stable_sort(o.parts.begin(), o.parts.end(),
+                     [](const Partition &x, const Partition &y) {
+                       return x.stages.size() > y.stages.size();
+                     });
+    out.stages.push_back(std::move(o));
+  }
 }
 
 // ---------------------------------------------------------------------------

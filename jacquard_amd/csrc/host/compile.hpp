@@ -94,11 +94,25 @@ struct Script {
 struct Compiled {
   std::vector<MajorStage> stages;
   std::vector<std::vector<uint8_t>> part_x;  // per stage, per partition
+  uint32_t level_split = 1;                  // split period actually used
+  CompileOptions opt;                        // options it was compiled with
 };
 
 // Level-split + partition + map.  Throws std::runtime_error on failure.
 void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
                         std::string *log = nullptr);
+
+// Partition merge (the step of src/pe.rs:855-1063, with the acceptance
+// decision left to the caller): `pairs` holds n_pairs (major, i, j) triples
+// naming two partitions of one major stage of `in`; each pair is mapped as
+// one partition over the union of their endpoints.  Pairs of a stage must
+// be disjoint.  A pair whose union does not fit one partition keeps its two
+// partitions.  `out` receives the design with every fitting pair replaced
+// by its merged partition (partitions re-sorted by decreasing boomerang
+// stages, as compile_partitions leaves them); merged[k] = 1 when pair k
+// merged.  Throws std::runtime_error on a malformed pair list.
+void merge_partitions(const Aig &g, const Compiled &in, const uint32_t *pairs,
+                      size_t n_pairs, Compiled &out, std::vector<uint8_t> &merged);
 
 // Block assignment ("masonry", src/flatten.rs:1207-1226) with per-partition
 // costs (nullptr = the reference heuristic stages+3) and flatten into the

@@ -36,6 +36,35 @@ class Compiled:
         self._h = h
         self.log = N.host().jqh_last_log().decode()
 
+    @classmethod
+    def _wrap(cls, netlist, handle, log=""):
+        c = cls.__new__(cls)
+        c.netlist = netlist
+        c._h = handle
+        c.log = log
+        return c
+
+    def merged(self, pairs):
+        """A new Compiled with each (major, i, j) pair of partitions mapped
+        as one partition over their joint endpoints (pairs of one major
+        stage must be disjoint); a pair that does not fit one partition
+        stays two.  Returns (compiled, merged bool[len(pairs)]).  Partition
+        indices of the result follow its own order (decreasing boomerang
+        stages), not this design's."""
+        p = np.ascontiguousarray(np.asarray(pairs, np.uint32).reshape(-1, 3))
+        ok = np.zeros(max(len(p), 1), np.uint8)
+        h = N.host().jqh_compiled_merge(self.netlist.handle, self._h, p.ctypes.data,
+                                        len(p), ok.ctypes.data)
+        if not h:
+            raise RuntimeError("merge failed: " + N.host_error())
+        return Compiled._wrap(self.netlist, h, self.log), ok[:len(p)].astype(bool)
+
+    def endpoint_counts(self):
+        """Endpoints per partition, [major][partition]."""
+        L = N.host()
+        return [[L.jqh_compiled_part_endpoints(self._h, m, p) for p in range(n)]
+                for m, n in enumerate(self.parts_per_stage())]
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:

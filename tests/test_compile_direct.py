@@ -97,3 +97,48 @@ def test_sram_storage_layout():
     s = comp.flatten(4)
     assert s.sram_storage_size == 2 * SRAM_WORDS
     assert sorted(s.sram_offsets.tolist()) == [0, SRAM_WORDS]
+
+
+@pytest.mark.parametrize("xprop", [False, True])
+def test_merged_partitions_vs_direct(xprop):
+    """Partition merge (Compiled.merged): every stage's partitions merged in
+    pairs where they fit one partition; the merged design has fewer
+    partitions, passes the validator, and its interpreted values (and X)
+    equal the direct AIG simulation."""
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    nl = Netlist.random(seed=21, gates=20000, depth=30, dffs=1500, inputs=48, outputs=24,
+                        geometric_p=0.3, srams=2)
+    comp = Compiled(nl, level_split=5, max_level_nodes=100)  # ~280 narrow partitions
+    pps = comp.parts_per_stage()
+    pairs = [(m, i, i + 1) for m, n in enumerate(pps) for i in range(0, n - 1, 2)]
+    assert pairs
+    mc, ok = comp.merged(pairs)
+    assert ok.sum() > 100
+    assert sum(mc.parts_per_stage()) == sum(pps) - int(ok.sum())
+    # every endpoint is still produced by exactly one partition
+    assert [sum(c) for c in mc.endpoint_counts()] == [sum(c) for c in comp.endpoint_counts()]
+    s = mc.flatten(4, xprop=xprop)
+    s.validate()
+    cycles = 50
+    m, pick = (None, None)
+    if xprop:
+        m, pick = stimulus.dff_initial_xmask(s, 0.1, seed=4)
+    st = stimulus.build_states(s, cycles, seed=6, p=0.3, initial_xmask=m)
+    sram = np.zeros(s.sram_storage_size, np.uint32)
+    sram_x = np.full(s.sram_storage_size, 0xFFFFFFFF, np.uint32) if xprop else None
+    oracle.run(s, st, cycles, sram=sram, sram_x=sram_x)
+    direct_check(s, st, cycles, 6, 0.3, xprop=xprop,
+                 dff_init_x=None if pick is None else pick.astype(np.uint8))
+
+
+def test_merge_rejects_malformed_pairs():
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    n = comp.parts_per_stage()[0]
+    with pytest.raises(RuntimeError):
+        comp.merged([(0, 0, n)])
+    with pytest.raises(RuntimeError):
+        comp.merged([(len(comp.parts_per_stage()), 0, 1)])
+    if n >= 3:
+        with pytest.raises(RuntimeError):
+            comp.merged([(0, 0, 1), (0, 1, 2)])

@@ -107,3 +107,42 @@ def test_learned_costs_against_heuristic():
     print(f"MAPE heuristic {mape_h:.3f} learned {mape_l:.3f}; us/cycle {r}", flush=True)
     assert mape_l < mape_h
     assert r["learned"] <= 1.03 * r["reference"]
+
+
+@pytest.mark.gpu
+def test_learned_merge_and_assignment_beat_the_heuristic():
+    """On a script whose blocks run several partitions per major stage (12
+    blocks for ~40 partitions per stage, as the 16M-gate workload runs about
+    4.6), the learned model's merge decision plus its block assignment run
+    faster than the heuristic assignment by more than the run-to-run spread
+    of either, over 5 interleaved repeats; the simulated values of the
+    merged design stay exact (checked against the oracle)."""
+    import oracle
+    from helpers import design
+    from jacquard_amd import batched as B
+    from jacquard_amd import costmodel as CM
+    from jacquard_amd import stimulus
+    nl, comp = design("cm", level_split=4, seed=2, gates=300000, depth=48, dffs=20000,
+                      inputs=256, outputs=128, geometric_p=0.3, invert_prob=0.5)
+    W, G = 12, 2
+    costs, model, info = CM.learn_costs(comp, W, G)
+    merged, mcosts, minfo = CM.learned_merge(comp, model, W, groups=G)
+    assert merged is not comp and minfo["pairs_kept"] > 0
+    spread = {}
+    r = CM.compare_assignments(comp, W, G, {"heuristic": None, "learned+merge": (merged, mcosts)},
+                               repeats=5, cycles=32, spread=spread)
+    gain = r["heuristic"] - r["learned+merge"]
+    print(f"us/cycle {r}, spread {spread}, merge {minfo}", flush=True)
+    assert gain > max(spread["heuristic"], spread["learned+merge"]), (r, spread)
+    # exact values: lanes of the merged, learned-assignment script against
+    # the oracle on the original script
+    s = merged.flatten(W, costs=mcosts)
+    s0 = comp.flatten(W)
+    seeds, probs, cyc = np.array([5, 9], np.uint32), np.array([0.2, 0.4]), np.array([24, 17])
+    run = B.BatchedRun(s, seeds, probs, cyc, groups=1)
+    out = run.run()
+    for lane in range(2):
+        st = stimulus.build_states(s0, int(cyc[lane]), seed=int(seeds[lane]), p=float(probs[lane]))
+        oracle.run(s0, st, int(cyc[lane]))
+        want = B.plain_observations(s0, st, int(cyc[lane]), run.obs_lits)
+        assert np.array_equal(out[lane], want)
