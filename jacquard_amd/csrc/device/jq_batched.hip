@@ -45,7 +45,9 @@ namespace {
 
 constexpr int kLayerWords = 8192;
 constexpr int kSecWords = 5120;  // 4096 permutation words + 256 x 4 flags
-constexpr int kLdsWords = 2 * kLayerWords + 2 * kSecWords;
+// LDS: the layer, the writeout area, two script-section buffers and the
+// commit's old writeout words (136 KB of the 160 KB)
+constexpr int kLdsWords = 3 * kLayerWords + 2 * kSecWords;
 constexpr u32 kNoSource = 0xFFFFFFFFu;
 
 struct BArgs {
@@ -140,6 +142,18 @@ __device__ __forceinline__ void sec_fetch(const u32 *g, u32 *l) {
                                      16, 0, 0);
 }
 
+// Copy the n (a multiple of 32) words at g into LDS at l, 16 bytes per
+// lane, with the agent-coherent cache policy of ld_coherent (sc1): the
+// words may have been written by other workgroups of the lane group.
+__device__ __forceinline__ void coherent_fetch(const u32 *g, u32 *l, u32 n) {
+  const u32 t = threadIdx.x, w = t >> 6;
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (4 * (t + 256 * j) < n)
+      __builtin_amdgcn_global_load_lds(g + 4 * (t + 256 * j), l + 4 * (64 * w + 256 * j),
+                                       16, 0, 16);
+}
+
 // Wait until this wave's outstanding global loads (including the LDS
 // copies above) have landed: s_waitcnt vmcnt(0).  Every wave runs it before
 // the workgroup barrier that precedes the first read of a fetched section.
@@ -179,9 +193,16 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
   const u32 t = threadIdx.x;
   const u32 lane = t & 63;
   u32 *layer = lds, *WO = lds + kLayerWords;
+  u32 *OLD = lds + 2 * kLayerWords + 2 * kSecWords;
   // section buffer k & 1 (pointer arithmetic, not an array: no scratch)
   auto secb = [lds](u32 k) { return lds + 2 * kLayerWords + (k & 1u) * kSecWords; };
   size_t pi = 0;
+  // state lane-words of the next partition's first layer, loaded during
+  // the current partition when both are in this major stage (their sources
+  // are earlier major stages and the previous snapshot, complete since the
+  // barrier before this stage)
+  u32 vnext[32];
+  bool have_next = false;
   for (;; gsrc += kLayerWords) {
     if (h.S == 0) {  // no partition in this stage: fetch the next stage's first
       h = load_bhdr(nscript);
@@ -203,14 +224,24 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
     const u32 next_s = h.S;
 
     sec_fetch(secg, secb(0));
+    // the commit's old values (this partition's writeouts in the previous
+    // snapshot, which nothing changes during the cycle) go to LDS now and
+    // land while the partition runs
+    coherent_fetch(in + (size_t)io * 32, OLD, num_ios * 32);
     // ---- global read through the gather table: 32 independent loads of
-    // state words at entries prefetched one partition ahead
+    // state words at entries prefetched one partition ahead (or the words
+    // themselves, when the previous partition of this stage loaded them)
     {
       u32 v[32];
+      if (have_next) {
 #pragma unroll
-      for (int j = 0; j < 32; ++j) {
-        u32 e = src[j];
-        v[j] = e != kNoSource ? ld_coherent(((e >> 31) ? out : in) + (e & 0x7fffffffu)) : 0u;
+        for (int j = 0; j < 32; ++j) v[j] = vnext[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+          u32 e = src[j];
+          v[j] = e != kNoSource ? ld_coherent(((e >> 31) ? out : in) + (e & 0x7fffffffu)) : 0u;
+        }
       }
       // the successor's entries load while this partition runs (the table
       // is read-only)
@@ -306,6 +337,16 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
         for (u32 q = 0; q < 32; ++q) WO[sw(32 * t + q)] = layer[sw(src + q)];
       }
     }
+    // the next partition of this stage: its first-layer words (its gather
+    // entries arrived during this partition)
+    have_next = !is_last && next_s != 0;
+    if (have_next) {
+#pragma unroll
+      for (int j = 0; j < 32; ++j) {
+        u32 e = src[j];
+        vnext[j] = e != kNoSource ? ld_coherent(((e >> 31) ? out : in) + (e & 0x7fffffffu)) : 0u;
+      }
+    }
     __syncthreads();
 
     // ---- SRAM + duplicate permutation into the layer (free now)
@@ -363,14 +404,13 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
       const u32 *sec = secb(S + 1);
       const u32 *fl = sec + 4096;
       u32 *o = out + (size_t)io * 32;
-      const u32 *i0 = in + (size_t)io * 32;
       for (u32 q = t; q < num_ios * 32; q += 256) {
         uint4 f = *reinterpret_cast<const uint4 *>(fl + 4 * (q >> 5));
         // a bit whose enable is forced (set0) needs no gather: ck = inv
         u32 ck = fbit(f.x, q);
         if (!((f.y >> (q & 31)) & 1u)) ck ^= WO[sw(perm_at(sec, q))];
         u32 d = WO[sw(q)] ^ fbit(f.z, q);
-        st_coherent(o + q, (ld_coherent(i0 + q) & ~ck) | (d & ck));
+        st_coherent(o + q, (OLD[q] & ~ck) | (d & ck));
       }
     }
     __syncthreads();
