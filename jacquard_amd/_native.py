@@ -60,7 +60,7 @@ def host():
         "jqh_aig_module": (c_cp, [c_vp]),
         "jqh_aig_dff_name": (c_cp, [c_vp, c_u32]),
         "jqh_aig_set_delays": (None, [c_vp, c_vp]),
-        "jqh_compile": (c_vp, [c_vp, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32]),
+        "jqh_compile": (c_vp, [c_vp, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32]),
         "jqh_compiled_free": (None, [c_vp]),
         "jqh_compiled_num_major": (c_u32, [c_vp]),
         "jqh_compiled_num_parts": (c_u32, [c_vp, c_u32]),

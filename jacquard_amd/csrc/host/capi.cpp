@@ -182,12 +182,14 @@ void jqh_aig_set_delays(void *h, const uint16_t *delay) {
 // ---------------------------------------------------------------- compile
 void *jqh_compile(void *aig, uint32_t level_split, uint32_t max_inputs,
                   uint32_t max_outputs, uint32_t max_level_nodes,
-                  uint32_t max_part_nodes, uint32_t multilevel) {
+                  uint32_t max_part_nodes, uint32_t multilevel,
+                  uint32_t max_boomerang_stages) {
   auto *g = static_cast<Aig *>(aig);
   auto *c = new Compiled();
   CompileOptions o;
   o.level_split = level_split;
   o.multilevel = multilevel;
+  o.max_boomerang_stages = max_boomerang_stages;
   if (max_inputs) o.max_inputs = max_inputs;
   if (max_outputs) o.max_outputs = max_outputs;
   if (max_level_nodes) o.max_level_nodes = max_level_nodes;

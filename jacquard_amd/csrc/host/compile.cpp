@@ -817,7 +817,9 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
     std::function<void(MapCtx &, std::vector<uint32_t>, std::vector<Partition> &)> build =
         [&](MapCtx &cx, std::vector<uint32_t> eps, std::vector<Partition> &res) {
       Partition part;
-      if (map_group(g, opt, ms, infos, avail, eps, cx, part)) {
+      if (map_group(g, opt, ms, infos, avail, eps, cx, part) &&
+          (!opt.max_boomerang_stages || eps.size() <= 1 ||
+           part.stages.size() <= opt.max_boomerang_stages)) {
         res.push_back(std::move(part));
         return;
       }

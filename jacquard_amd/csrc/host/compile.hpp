@@ -61,6 +61,9 @@ struct CompileOptions {
   // highest tree level at which the mapper places AND gates (0: level-1
   // mapper only, one logic level per boomerang stage)
   uint32_t multilevel = 8;
+  // a partition mapped to more boomerang stages than this is split in two
+  // by endpoints, like one that does not fit (0: no limit)
+  uint32_t max_boomerang_stages = 0;
 };
 
 struct Script {

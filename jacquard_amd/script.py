@@ -24,13 +24,16 @@ class Compiled:
     per-partition cost vector."""
 
     def __init__(self, netlist: Netlist, level_split=0, max_inputs=0,
-                 max_outputs=0, max_level_nodes=0, max_part_nodes=0, multilevel=8):
+                 max_outputs=0, max_level_nodes=0, max_part_nodes=0, multilevel=8,
+                 max_boomerang_stages=0):
         """multilevel: highest boomerang-tree level at which AND gates are
-        placed (0 = one logic level per boomerang stage)."""
+        placed (0 = one logic level per boomerang stage);
+        max_boomerang_stages: partitions mapped to more boomerang stages
+        are split by endpoints (0 = no limit)."""
         self.netlist = netlist
         h = N.host().jqh_compile(netlist.handle, level_split, max_inputs,
                                  max_outputs, max_level_nodes, max_part_nodes,
-                                 multilevel)
+                                 multilevel, max_boomerang_stages)
         if not h:
             raise RuntimeError("compile failed: " + N.host_error())
         self._h = h

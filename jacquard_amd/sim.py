@@ -141,7 +141,10 @@ def tune_plain(netlist, level_split, caps=(2600, 1800, 1300, 1100, 900, 800, 700
     partitions with fewer boomerang stages each, up to the point where the
     widest major stage no longer fits the co-resident workgroups.  Every cap
     whose widest stage fits is compiled and timed over ``trial_cycles``
-    cycles; the fastest wins.  Returns (Compiled, script, table)."""
+    cycles, then the best cap again with every partition held to one
+    boomerang stage (``max_boomerang_stages=1``: partitions the mapper
+    gives two stages are split by endpoints); the fastest wins.  Returns
+    (Compiled, script, table)."""
     from .script import Compiled
     from . import stimulus
     torch = _torch()
@@ -149,11 +152,21 @@ def tune_plain(netlist, level_split, caps=(2600, 1800, 1300, 1100, 900, 800, 700
     N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
     stream = torch.cuda.current_stream()
     best, table = None, []
-    for cap in caps:
-        comp = Compiled(netlist, level_split=level_split, max_level_nodes=cap)
+    # every cap as compiled, then the best cap with every partition held to
+    # one boomerang stage (a stage's slowest partition sets its time, and
+    # the two-stage ones are the slowest)
+    variants = [(cap, 0) for cap in caps] + [(None, 1)]
+    for cap, one_stage in variants:
+        if cap is None:
+            if best is None:
+                break
+            cap = best[3]
+        comp = Compiled(netlist, level_split=level_split, max_level_nodes=cap,
+                        max_boomerang_stages=one_stage)
         widest = max(comp.parts_per_stage())
         if widest > res.value:
-            table.append({"max_level_nodes": cap, "widest_stage": widest, "fits": False})
+            table.append({"max_level_nodes": cap, "max_boomerang_stages": one_stage,
+                          "widest_stage": widest, "fits": False})
             continue
         s = comp.flatten(widest)
         ss = s.effective_state_size()
@@ -178,13 +191,15 @@ def tune_plain(netlist, level_split, caps=(2600, 1800, 1300, 1100, 900, 800, 700
         if device_error():
             raise RuntimeError("device barrier timeout while tuning the plain script")
         rate = trial_cycles / (e0.elapsed_time(e1) / 1e3)
-        table.append({"max_level_nodes": cap, "widest_stage": widest, "fits": True,
+        table.append({"max_level_nodes": cap, "max_boomerang_stages": one_stage,
+                      "widest_stage": widest, "fits": True,
                       "partitions": s.num_parts, "cycles_per_s": rate})
         if log:
-            log(f"plain tune: max_level_nodes {cap}: {s.num_parts} partitions, widest stage "
-                f"{widest}, {rate:,.0f} cycles/s")
+            log(f"plain tune: max_level_nodes {cap}"
+                + (", one boomerang stage per partition" if one_stage else "")
+                + f": {s.num_parts} partitions, widest stage {widest}, {rate:,.0f} cycles/s")
         if best is None or rate > best[2]:
-            best = (comp, s, rate)
+            best = (comp, s, rate, cap)
         del st, d_bs, d_bd, one
     if best is None:
         raise RuntimeError("no partition width fits the co-resident workgroups")

@@ -142,3 +142,26 @@ def test_merge_rejects_malformed_pairs():
     if n >= 3:
         with pytest.raises(RuntimeError):
             comp.merged([(0, 0, 1), (0, 1, 2)])
+
+
+def test_one_boomerang_stage_per_partition_vs_direct():
+    """max_boomerang_stages=1 splits every partition the mapper would give
+    two or more boomerang stages, down to single endpoints (one endpoint
+    whose cone is deeper than a stage keeps its stages); the values stay
+    those of the direct AIG simulation."""
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    nl = Netlist.random(seed=23, gates=20000, depth=40, dffs=1500, inputs=48, outputs=24,
+                        geometric_p=0.3, srams=1)
+    free = Compiled(nl, level_split=10)
+    assert max(max(c) for c in free.stage_counts()) > 1
+    comp = Compiled(nl, level_split=10, max_boomerang_stages=1)
+    for stages, eps in zip(comp.stage_counts(), comp.endpoint_counts()):
+        assert all(st == 1 or ep == 1 for st, ep in zip(stages, eps))
+    assert sum(comp.parts_per_stage()) > sum(free.parts_per_stage())
+    s = comp.flatten(8)
+    s.validate()
+    cycles = 40
+    st = stimulus.build_states(s, cycles, seed=8, p=0.3)
+    oracle.run(s, st, cycles, sram=np.zeros(s.sram_storage_size, np.uint32))
+    direct_check(s, st, cycles, 8, 0.3)
