@@ -27,7 +27,11 @@ _hip = None
 
 
 def _load(name):
-    path = os.path.join(LIBDIR, name)
+    # JACQUARD_HOST_LIB_DIR points the host library at another build of the
+    # same sources (the sanitizer build of the CPU tests); the HIP library
+    # always comes from the in-tree build
+    d = os.environ.get("JACQUARD_HOST_LIB_DIR") if name == "libjacquard_host.so" else None
+    path = os.path.join(d or LIBDIR, name)
     if not os.path.exists(path):
         raise NativeLibraryMissing(
             f"{path} is not built; run `python -m jacquard_amd.build` "

@@ -29,7 +29,8 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
-    path = os.path.join(HERE, "libjq_oracle.so")
+    d = os.environ.get("JACQUARD_ORACLE_LIB_DIR")  # the sanitizer build
+    path = os.path.join(d or HERE, "libjq_oracle.so")
     if not os.path.exists(path):
         build()
     L = ctypes.CDLL(path)
