@@ -256,6 +256,7 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
     sec_wait();
     __syncthreads();
 
+    const u32 nperm = ns * 4 + nd;  // SRAM port and duplicate words
     // ---- boomerang stages, in place on the one layer: every thread first
     // gathers its 32 level-1 operands into registers, then (after a
     // barrier) the layer is overwritten with this stage's after-layout
@@ -264,8 +265,11 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
       const uint4 P0 = S4[t], P1 = S4[256 + t], P2 = S4[512 + t], P3 = S4[768 + t];
       const uint4 F = S4[1024 + t];
       // the next section goes into the other buffer, last read by the
-      // previous stage, which finished before the last barrier
-      sec_fetch(secg + (size_t)kSecWords * (s + 1), secb(s + 1));
+      // previous stage, which finished before the last barrier; after the
+      // last stage that is the SRAM/dup section, or the clock-enable
+      // section when the partition has no SRAM or duplicate words
+      const u32 nxt = (s + 1 == S && nperm == 0) ? S + 1 : s + 1;
+      sec_fetch(secg + (size_t)kSecWords * nxt, secb(s + 1));
       const u32 pw[16] = {P0.x, P0.y, P0.z, P0.w, P1.x, P1.y, P1.z, P1.w,
                           P2.x, P2.y, P2.z, P2.w, P3.x, P3.y, P3.z, P3.w};
       u32 g[32];
@@ -349,59 +353,63 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
     }
     __syncthreads();
 
-    // ---- SRAM + duplicate permutation into the layer (free now)
-    const u32 nperm = ns * 4 + nd;
-    {
-      const u32 *sec = secb(S);
-      const u32 *fl = sec + 4096;
-      // the clock-enable section goes into the buffer of the last stage
-      sec_fetch(secg + (size_t)kSecWords * (S + 1), secb(S + 1));
-      for (u32 q = t; q < nperm * 32; q += 256) {
-        uint4 f = *reinterpret_cast<const uint4 *>(fl + 4 * (q >> 5));
-        u32 v = WO[sw(perm_at(sec, q))];
-        layer[sw(q)] = (v & ~fbit(f.y, q)) ^ fbit(f.x, q);
-      }
-    }
-    __syncthreads();
-    // SRAM ports: thread per (sram, lane)
-    for (u32 base = 0; base < ns * 32; base += 256) {
-      u32 id = base + t;
-      u32 s = id >> 5, l = id & 31;
-      u32 r = 0;
-      bool act = id < ns * 32;
-      if (act) {
-        const u32 P = 128 * s;
-        u32 ra = 0, wa = 0, en = 0, data = 0;
-        for (int k = 0; k < 13; ++k) {
-          ra |= ((layer[sw(P + k)] >> l) & 1u) << k;
-          wa |= ((layer[sw(P + 16 + k)] >> l) & 1u) << k;
+    // ---- SRAM + duplicate permutation into the layer (free now); a
+    // partition without SRAM or duplicate words skips the phase and its
+    // barriers (the condition is uniform), its clock-enable section already
+    // in place
+    if (nperm) {
+      {
+        const u32 *sec = secb(S);
+        const u32 *fl = sec + 4096;
+        // the clock-enable section goes into the buffer of the last stage
+        sec_fetch(secg + (size_t)kSecWords * (S + 1), secb(S + 1));
+        for (u32 q = t; q < nperm * 32; q += 256) {
+          uint4 f = *reinterpret_cast<const uint4 *>(fl + 4 * (q >> 5));
+          u32 v = WO[sw(perm_at(sec, q))];
+          layer[sw(q)] = (v & ~fbit(f.y, q)) ^ fbit(f.x, q);
         }
+      }
+      __syncthreads();
+      // SRAM ports: thread per (sram, lane)
+      for (u32 base = 0; base < ns * 32; base += 256) {
+        u32 id = base + t;
+        u32 s = id >> 5, l = id & 31;
+        u32 r = 0;
+        bool act = id < ns * 32;
+        if (act) {
+          const u32 P = 128 * s;
+          u32 ra = 0, wa = 0, en = 0, data = 0;
+          for (int k = 0; k < 13; ++k) {
+            ra |= ((layer[sw(P + k)] >> l) & 1u) << k;
+            wa |= ((layer[sw(P + 16 + k)] >> l) & 1u) << k;
+          }
+          for (int k = 0; k < 32; ++k) {
+            en |= ((layer[sw(P + 32 + k)] >> l) & 1u) << k;
+            data |= ((layer[sw(P + 64 + k)] >> l) & 1u) << k;
+          }
+          u32 *ram = ram_group + (size_t)l * A.sram_size + so + (size_t)s * kSramWords;
+          r = ld_coherent(ram + ra);
+          u32 w0 = ld_coherent(ram + wa);
+          st_coherent(ram + wa, (w0 & ~en) | (data & en));
+        }
+        // transpose the 32 read words of each SRAM into lane-words
+        u32 mine = 0;
         for (int k = 0; k < 32; ++k) {
-          en |= ((layer[sw(P + 32 + k)] >> l) & 1u) << k;
-          data |= ((layer[sw(P + 64 + k)] >> l) & 1u) << k;
+          unsigned long long m = __ballot(act && ((r >> k) & 1u));
+          u32 half = (u32)(m >> (32 * ((t >> 5) & 1)));
+          if (l == (u32)k) mine = half;
         }
-        u32 *ram = ram_group + (size_t)l * A.sram_size + so + (size_t)s * kSramWords;
-        r = ld_coherent(ram + ra);
-        u32 w0 = ld_coherent(ram + wa);
-        st_coherent(ram + wa, (w0 & ~en) | (data & en));
+        if (act) WO[sw(32 * (num_ios - ns + s) + l)] = mine;
       }
-      // transpose the 32 read words of each SRAM into lane-words
-      u32 mine = 0;
-      for (int k = 0; k < 32; ++k) {
-        unsigned long long m = __ballot(act && ((r >> k) & 1u));
-        u32 half = (u32)(m >> (32 * ((t >> 5) & 1)));
-        if (l == (u32)k) mine = half;
-      }
-      if (act) WO[sw(32 * (num_ios - ns + s) + l)] = mine;
+      for (u32 q = t; q < nd * 32; q += 256)
+        WO[sw(32 * (num_ios - ns - nd) + q)] = layer[sw(ns * 128 + q)];
+      sec_wait();  // the clock-enable section is read after this barrier
+      __syncthreads();
     }
-    for (u32 q = t; q < nd * 32; q += 256)
-      WO[sw(32 * (num_ios - ns - nd) + q)] = layer[sw(ns * 128 + q)];
-    sec_wait();  // the clock-enable section is read after this barrier
-    __syncthreads();
 
     // ---- clock-enable permutation + commit (coalesced lane-words)
     {
-      const u32 *sec = secb(S + 1);
+      const u32 *sec = secb(nperm ? S + 1 : S);
       const u32 *fl = sec + 4096;
       u32 *o = out + (size_t)io * 32;
       for (u32 q = t; q < num_ios * 32; q += 256) {
