@@ -15,7 +15,8 @@ resumable launch over an equal share of the job's group cycles.  Beside it:
 the plain drop-in on the same netlist, the reference CPU path (the
 oracle's literal restatement of sanity_check_cpu, one host core, a bounded
 cycle sample), and under config.forms the 1M-gate plain drop-in over 100K
-cycles and the 256 batched lanes of the 1M-gate netlist.  `roofline` is the
+cycles, the 256 batched lanes of the 1M-gate netlist and the structured
+16M-gate netlist's 64 lanes x 20K cycles with learned costs.  `roofline` is the
 batched kernel's: algorithmic HBM bytes per lane-cycle over its kernel time
 (HIP events on the launch stream), with the counter-measured traffic of the
 committed FETCH_SIZE / WRITE_SIZE profile of this workload beside it.
@@ -59,7 +60,9 @@ def parse():
                     help="north: launches of --cycles-per-step cycles of the plain "
                          "drop-in on the 4M-gate netlist")
     ap.add_argument("--no-forms", action="store_true",
-                    help="north: skip the 1M-gate plain and batched forms")
+                    help="north: skip the 1M- and 16M-gate forms")
+    ap.add_argument("--no-16m", action="store_true",
+                    help="north: skip the 16M-gate structured netlist's form")
     ap.add_argument("--workload", choices=["north", "plain", "xtiming", "lanes4m",
                                            "mult16m", "cosim"],
                     default="north",
@@ -255,7 +258,7 @@ def bench_plain(a, script, torch):
 
 
 def bench_batched(a, comp, torch, cycles=None, split=None, steps=0, learned=None,
-                  widths=None):
+                  widths=None, merge=True, seeds_probs=None):
     """Batched form on the same netlist: --lanes lanes with per-lane seeds,
     cycle counts U[10K, 200K] x --batched-scale and toggle rates
     U[0.02, 0.5], in one persistent launch with the device lane queue.
@@ -275,6 +278,8 @@ def bench_batched(a, comp, torch, cycles=None, split=None, steps=0, learned=None
     lo, hi = int(10_000 * a.batched_scale), int(200_000 * a.batched_scale)
     seeds = rng.integers(1, 2**31, a.lanes).astype(np.uint32)
     probs = rng.uniform(0.02, 0.5, a.lanes)
+    if seeds_probs is not None:
+        seeds, probs = seeds_probs
     if cycles is None:
         cycles = rng.integers(max(lo, 1), max(hi, 1) + 1, a.lanes)
     if learned is None:
@@ -302,10 +307,12 @@ def bench_batched(a, comp, torch, cycles=None, split=None, steps=0, learned=None
         # partition merges; measured kernel time picks among the heuristic,
         # the learned assignment and the learned merge + assignment
         costs, model, info = CM.learn_costs(comp, W, G, log=log)
-        merged, mcosts, minfo = CM.learned_merge(comp, model, W, groups=G, log=log)
         variants = {"heuristic": None, "learned": costs}
-        if merged is not comp:
-            variants["learned+merge"] = (merged, mcosts)
+        minfo = None
+        if merge:
+            merged, mcosts, minfo = CM.learned_merge(comp, model, W, groups=G, log=log)
+            if merged is not comp:
+                variants["learned+merge"] = (merged, mcosts)
         spread = {}
         ab = CM.compare_assignments(comp, W, G, variants, repeats=3, cycles=32,
                                     spread=spread)
@@ -563,8 +570,10 @@ def bench_north(a, torch):
     time of the whole job, run as --steps graph replays (after --warmup
     short launches).  Beside it: the plain drop-in on the same netlist
     (--plain-launches x --cycles-per-step cycles), and under config.forms
-    the 1M-gate plain drop-in (100K cycles) and the 256 batched lanes of the
-    1M-gate netlist (one graph replay), unless --no-forms."""
+    the 1M-gate plain drop-in (100K cycles), the 256 batched lanes of the
+    1M-gate netlist (one graph replay) and the structured 16M-gate
+    netlist's 64 lanes x 20K cycles with learned costs, unless --no-forms
+    (--no-16m drops the last)."""
     import ctypes
     from jacquard_amd import _native as N
     from jacquard_amd.script import Compiled
@@ -588,6 +597,8 @@ def bench_north(a, torch):
     forms = {}
     if not a.no_forms:
         forms = forms_1m(a, torch)
+        if not a.no_16m:
+            forms.update(forms_16m(a, torch))
     cpu = None
     if not a.no_cpu:
         model, online, usable = host_cpu()
@@ -754,6 +765,51 @@ def bench_mult16m(a, torch):
                      "algorithmic_bytes_per_lane_cycle": per_lane},
         "cpu_baseline": None,
     }), flush=True)
+
+
+def forms_16m(a, torch):
+    """The structured 16M-gate netlist reported beside the headline: 64 lanes
+    x 20,000 cycles through 2 groups x 128 blocks with the learned block
+    assignment (A/B against the heuristic; no merge and no tiling sweep,
+    whose flattenings of a 0.9 GB script take tens of seconds each) in one
+    graph replay, and the plain drop-in over 2 x 1,000 cycles."""
+    import ctypes
+    from jacquard_amd import _native as N
+    from jacquard_amd import batched as B
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    f = argparse.Namespace(**vars(a))
+    f.lanes, f.batched_groups, f.no_width_tune = 64, 2, True
+    f.steps, f.warmup, f.cycles_per_step = 2, 1, 1000
+    t = time.time()
+    nl = Netlist.multiplier(seed=a.seed)
+    comp = Compiled(nl, level_split=8)
+    t_host = time.time() - t
+    log(f"forms: 16M netlist {nl.counts()} generated and compiled in {t_host:.1f}s")
+    f.batched_blocks = max(8, min(max(comp.parts_per_stage()),
+                                  B.max_resident_batched() // f.batched_groups))
+    rng = np.random.default_rng(a.seed + 200)
+    sp = (rng.integers(1, 2**31, 64).astype(np.uint32), rng.uniform(0.02, 0.5, 64))
+    b = bench_batched(f, comp, torch, cycles=np.full(64, 20_000), split=8, steps=0,
+                      learned=True, merge=False, seeds_probs=sp)
+    value = b["lane_cycles"] / b["wall"]
+    res = ctypes.c_int(0)
+    N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
+    script = comp.flatten(min(max(comp.parts_per_stage()), res.value))
+    p = bench_plain(f, script, torch)
+    plain = p["cycles"] / p["wall"]
+    log(f"forms: 16M batched {value:,.0f} lane-cycles/s, plain {plain:,.0f} cycles/s")
+    return {"batched_16m": {
+        "workload": f"structured netlist: {nl.counts()['num_and']} AND (64 16x16 "
+                    f"multiplier tiles with pipeline DFFs + 2M-gate control AIG), "
+                    f"{nl.counts()['num_dffs']} DFF, level split 8; 64 lanes x 20000 "
+                    f"cycles, toggle U[0.02, 0.5]; one hipGraph replay",
+        "lane_cycles_per_s": value, "groups": b["groups"], "blocks_per_group": b["blocks"],
+        "lane_cycles": b["lane_cycles"], "learned_cost_model": b["learned"],
+        "partitions": b["script"].num_parts, "script_bytes": int(b["script"].blocks_data.nbytes),
+        "host_generate_compile_s": t_host, "verified_cycles": b["verified_cycles"],
+        "plain_lane_cycles_per_s": plain, "plain_blocks": script.num_blocks,
+        "plain_verified_cycles": p.get("verified_cycles"), "vs_plain": value / plain}}
 
 
 def bench_cosim(a, torch, comp=None, emit=True):
