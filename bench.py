@@ -312,7 +312,7 @@ def bench_batched(a, comp, torch, cycles=None, split=None, steps=0, learned=None
         if merge:
             merged, mcosts, minfo = CM.learned_merge(comp, model, W, groups=G, log=log)
             if merged is not comp:
-                variants["learned+merge"] = (merged, mcosts)
+                variants["learned+cut/merge"] = (merged, mcosts)
         spread = {}
         ab = CM.compare_assignments(comp, W, G, variants, repeats=3, cycles=32,
                                     spread=spread)
@@ -323,7 +323,7 @@ def bench_batched(a, comp, torch, cycles=None, split=None, steps=0, learned=None
             + f"; using {best}")
         if best == "heuristic":
             costs = None  # the heuristic stays the fallback; measured time judges
-        elif best == "learned+merge":
+        elif best == "learned+cut/merge":
             comp, costs = merged, mcosts
     script = comp.flatten_shared(W) if costs is None else comp.flatten(W, costs=costs)
     run = B.BatchedRun(script, seeds, probs, cycles, groups=G)

@@ -71,6 +71,7 @@ def host():
         "jqh_compiled_part_stages": (c_u32, [c_vp, c_u32, c_u32]),
         "jqh_compiled_part_endpoints": (c_u32, [c_vp, c_u32, c_u32]),
         "jqh_compiled_merge": (c_vp, [c_vp, c_vp, c_vp, c_u64, c_vp]),
+        "jqh_compiled_split": (c_vp, [c_vp, c_vp, c_vp, c_u64, c_vp]),
         "jqh_flatten": (c_vp, [c_vp, c_vp, c_u32, ctypes.c_int, c_vp]),
         "jqh_script_free": (None, [c_vp]),
         "jqh_script_info": (None, [c_vp, P(c_u64)]),

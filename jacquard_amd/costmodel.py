@@ -251,35 +251,63 @@ def merge_candidates(pred, blocks):
     return pairs
 
 
-def learned_merge(compiled, model, blocks, groups=None, sample_cycles=16, epochs=100,
-                  log=None):
-    """The partition-merge step driven by the learned latency model (the
-    role of the merge acceptance of src/pe.rs:1033-1046).  In each major
-    stage with more partitions than blocks, the candidates of
-    merge_candidates are mapped as merged partitions; the model predicts
-    the merged partitions' latencies from their script sections, and a
-    stage keeps its merges when its predicted makespan (largest block total
-    after the longest-first assignment of predicted costs) drops.
+def split_candidates(pred, endpoints, blocks):
+    """Partitions (major, i) proposed for a cut: in every major stage with
+    n < blocks partitions, the blocks - n of largest predicted latency that
+    have more than one endpoint (the stage's time is its slowest block, and
+    the free blocks can take the halves)."""
+    out = []
+    for m, c in enumerate(pred):
+        k = blocks - len(c)
+        if k <= 0:
+            continue
+        order = np.argsort(-np.asarray(c), kind="stable")
+        out += [(m, int(i)) for i in order if endpoints[m][i] > 1][:k]
+    return out
 
-    Merged partitions are larger than those the model was trained on, so
-    with `groups` given (and the model's training set kept by learn_costs)
-    the trial design is profiled first (sample_cycles cycles of `groups`
-    lane groups) and the model refitted on the old and new samples before
-    it predicts.  Returns (compiled design with the kept merges, its learned
-    costs, info); with nothing kept, the design is `compiled` itself."""
+
+def learned_merge(compiled, model, blocks, groups=None, sample_cycles=16, epochs=100,
+                  log=None, cut=True):
+    """The partition cut and merge step driven by the learned latency model
+    (the role of the merge acceptance of src/pe.rs:1033-1046, and of the
+    partition sizing that precedes it).  In each major stage with more
+    partitions than blocks, the candidates of merge_candidates are mapped
+    as merged partitions; with cut, in each stage with fewer, the slowest
+    predicted partitions of split_candidates are mapped as two halves each.
+    The model predicts the new partitions' latencies from their script
+    sections, and a stage keeps its changes when its predicted makespan
+    (largest block total after the longest-first assignment of predicted
+    costs) drops.
+
+    The new partitions' sizes lie outside those the model was trained on,
+    so with `groups` given (and the model's training set kept by
+    learn_costs) the trial design is profiled first (sample_cycles cycles
+    of `groups` lane groups) and the model refitted on the old and new
+    samples before it predicts.  Returns (compiled design with the kept
+    changes, its learned costs, info); with nothing kept, the design is
+    `compiled` itself."""
     s0 = compiled.flatten_shared(blocks)
     pred0 = stage_costs(compiled, s0, model.predict(s0.part_features()))
     before = [lpt_makespan(c, blocks) for c in pred0]
-    info = dict(candidates=0, merged=0, stages_kept=0, refit_samples=0,
-                predicted_ticks_before=float(sum(before)))
+    info = dict(candidates=0, merged=0, cut_candidates=0, cut=0, stages_kept=0,
+                refit_samples=0, predicted_ticks_before=float(sum(before)))
     pairs = merge_candidates(pred0, blocks)
-    info["candidates"] = len(pairs)
-    if not pairs:
+    cuts = split_candidates(pred0, compiled.endpoint_counts(), blocks) if cut else []
+    info["candidates"], info["cut_candidates"] = len(pairs), len(cuts)
+    if not pairs and not cuts:
         info["predicted_ticks_after"] = info["predicted_ticks_before"]
         return compiled, pred0, info
-    trial, ok = compiled.merged(pairs)
+
+    def build(p, c):
+        # cuts first (their stages have no merge pairs, so the merge pairs'
+        # partition indices stay valid: untouched stages keep their order)
+        d, okc = compiled.split(c) if c else (compiled, np.zeros(0, bool))
+        d, okm = d.merged(p) if p else (d, np.zeros(0, bool))
+        return d, okm, okc
+
+    trial, ok, okc = build(pairs, cuts)
     s1 = trial.flatten_shared(blocks)
-    if groups and ok.any() and getattr(model, "train_x", None) is not None:
+    if groups and (ok.any() or okc.any()) and getattr(model, "train_x", None) is not None:
         _s, ticks = profile_batched(trial, blocks, groups, sample_cycles + 2, seed=7)
         t = ticks[2:]
         f1 = s1.part_features()
@@ -295,19 +323,21 @@ def learned_merge(compiled, model, blocks, groups=None, sample_cycles=16, epochs
     after = [lpt_makespan(c, blocks) for c in pred1]
     keep = {m for m in range(len(pred0)) if after[m] < before[m]}
     kept = [p for p, o in zip(pairs, ok) if o and p[0] in keep]
-    info.update(merged=int(ok.sum()), stages_kept=len(keep), pairs_kept=len(kept),
+    kept_cuts = [c for c, o in zip(cuts, okc) if o and c[0] in keep]
+    info.update(merged=int(ok.sum()), cut=int(okc.sum()), stages_kept=len(keep),
+                pairs_kept=len(kept), cuts_kept=len(kept_cuts),
                 predicted_ticks_after=float(sum(after[m] if m in keep else before[m]
                                                 for m in range(len(pred0)))))
     if log:
-        log(f"learned merge: {len(pairs)} candidate pairs, {int(ok.sum())} fit one "
-            f"partition; {len(keep)} of {len(pred0)} major stages keep theirs; predicted "
-            f"ticks per cycle {info['predicted_ticks_before']:.0f} -> "
-            f"{info['predicted_ticks_after']:.0f}")
-    if not kept:
+        log(f"learned cut/merge: {len(pairs)} merge pairs ({int(ok.sum())} fit one "
+            f"partition), {len(cuts)} cuts ({int(okc.sum())} fit); {len(keep)} of "
+            f"{len(pred0)} major stages keep theirs; predicted ticks per cycle "
+            f"{info['predicted_ticks_before']:.0f} -> {info['predicted_ticks_after']:.0f}")
+    if not kept and not kept_cuts:
         return compiled, pred0, info
-    if len(kept) == int(ok.sum()):
+    if len(kept) == int(ok.sum()) and len(kept_cuts) == int(okc.sum()):
         return trial, pred1, info
-    final, _ = compiled.merged(kept)
+    final, _, _ = build(kept, kept_cuts)
     sf = final.flatten_shared(blocks)
     return final, stage_costs(final, sf, model.predict(sf.part_features())), info
 

@@ -219,6 +219,22 @@ void *jqh_compiled_merge(void *aig, void *compiled, const uint32_t *pairs,
   return o;
 }
 
+// A new compiled design with the partitions (major, i) of `compiled` each
+// split in two (split_partitions); split_out[k] = 1 for each one split.
+void *jqh_compiled_split(void *aig, void *compiled, const uint32_t *parts,
+                         uint64_t n_parts, uint8_t *split_out) {
+  auto *g = static_cast<Aig *>(aig);
+  auto *c = static_cast<Compiled *>(compiled);
+  auto *o = new Compiled();
+  std::vector<uint8_t> split;
+  if (guard([&] { split_partitions(*g, *c, parts, n_parts, *o, split); })) {
+    delete o;
+    return nullptr;
+  }
+  for (size_t k = 0; k < split.size(); ++k) split_out[k] = split[k];
+  return o;
+}
+
 uint32_t jqh_compiled_part_endpoints(void *h, uint32_t m, uint32_t p) {
   return (uint32_t)static_cast<Compiled *>(h)->stages[m].parts[p].endpoints.size();
 }

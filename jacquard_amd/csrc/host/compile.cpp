@@ -981,6 +981,107 @@ void merge_partitions(const Aig &g, const Compiled &in, const uint32_t *pairs,
   }
 }
 
+void split_partitions(const Aig &g, const Compiled &in, const uint32_t *parts,
+                      size_t n_parts, Compiled &out, std::vector<uint8_t> &split) {
+  const uint32_t nmajor = (uint32_t)in.stages.size();
+  std::vector<std::vector<uint32_t>> by_stage(nmajor);
+  std::vector<std::vector<size_t>> part_id(nmajor);
+  for (size_t k = 0; k < n_parts; ++k) {
+    uint32_t m = parts[2 * k], i = parts[2 * k + 1];
+    if (m >= nmajor || i >= in.stages[m].parts.size())
+      throw std::runtime_error("split: partition out of range");
+    by_stage[m].push_back(i);
+    part_id[m].push_back(k);
+  }
+  Levels L = levelize(g);
+  const uint32_t lsplit = in.level_split ? in.level_split : 1;
+  auto stage_of = [&](uint32_t p) -> int {
+    if (g.drv[p] != kAnd) return -1;
+    return (int)std::min<uint32_t>((L.lvl[p] - 1) / lsplit, nmajor - 1);
+  };
+  out.stages.clear();
+  out.stages.reserve(nmajor);
+  out.part_x = in.part_x;
+  out.level_split = in.level_split;
+  out.opt = in.opt;
+  split.assign(n_parts, 0);
+  std::vector<std::unique_ptr<MapCtx>> ctxs;
+  for (uint32_t m = 0; m < nmajor; ++m) {
+    const MajorStage &ms = in.stages[m];
+    MajorStage o;
+    o.staged_out = ms.staged_out;
+    o.orig_endpoints = ms.orig_endpoints;
+    const auto &pl = by_stage[m];
+    std::vector<uint8_t> used(ms.parts.size(), 0);
+    for (uint32_t i : pl) {
+      if (used[i]) throw std::runtime_error("split: a partition is named twice");
+      used[i] = 1;
+    }
+    std::vector<EndpointInfo> infos;
+    if (!pl.empty()) {
+      infos.resize(ms.num_endpoints());
+      for (size_t e = 0; e < infos.size(); ++e) endpoint_outputs(g, ms, e, infos[e]);
+    }
+    const std::function<bool(uint32_t)> avail = [&](uint32_t p) {
+      return stage_of(p) < (int)m;
+    };
+    std::vector<Partition> lo(pl.size()), hi(pl.size());
+    std::vector<uint8_t> ok(pl.size(), 0);
+    std::string par_err;
+#pragma omp parallel
+    {
+      int tid = 0;
+#ifdef _OPENMP
+      tid = omp_get_thread_num();
+#endif
+#pragma omp single
+      {
+        int nt = 1;
+#ifdef _OPENMP
+        nt = omp_get_num_threads();
+#endif
+        if ((int)ctxs.size() < nt) ctxs.resize(nt);
+      }
+#pragma omp for schedule(dynamic, 1)
+      for (int64_t k = 0; k < (int64_t)pl.size(); ++k) {
+        try {
+          if (!ctxs[tid]) ctxs[tid] = std::make_unique<MapCtx>(g.num_pins + 1);
+          const auto &eps = ms.parts[pl[k]].endpoints;
+          if (eps.size() < 2) continue;
+          std::vector<uint32_t> a(eps.begin(), eps.begin() + eps.size() / 2);
+          std::vector<uint32_t> b(eps.begin() + eps.size() / 2, eps.end());
+          ok[k] = map_group(g, in.opt, ms, infos, avail, a, *ctxs[tid], lo[k]) &&
+                          map_group(g, in.opt, ms, infos, avail, b, *ctxs[tid], hi[k])
+                      ? 1
+                      : 0;
+        } catch (const std::exception &e) {
+#pragma omp critical(jq_split_err)
+          if (par_err.empty()) par_err = e.what();
+        }
+      }
+    }
+    if (!par_err.empty()) throw std::runtime_error(par_err);
+    std::vector<uint8_t> drop(ms.parts.size(), 0);
+    for (size_t k = 0; k < pl.size(); ++k)
+      if (ok[k]) {
+        drop[pl[k]] = 1;
+        split[part_id[m][k]] = 1;
+      }
+    for (size_t i = 0; i < ms.parts.size(); ++i)
+      if (!drop[i]) o.parts.push_back(ms.parts[i]);
+    for (size_t k = 0; k < pl.size(); ++k)
+      if (ok[k]) {
+        o.parts.push_back(std::move(lo[k]));
+        o.parts.push_back(std::move(hi[k]));
+      }
+    std::stable_sort(o.parts.begin(), o.parts.end(),
+                     [](const Partition &x, const Partition &y) {
+                       return x.stages.size() > y.stages.size();
+                     });
+    out.stages.push_back(std::move(o));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Flatten (src/flatten.rs:470-1130 for one partition, 1133-1573 overall).
 namespace {

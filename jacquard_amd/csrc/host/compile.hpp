@@ -117,6 +117,16 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
 void merge_partitions(const Aig &g, const Compiled &in, const uint32_t *pairs,
                       size_t n_pairs, Compiled &out, std::vector<uint8_t> &merged);
 
+// Partition cut, the converse: `parts` holds n_parts (major, i) pairs naming
+// partitions of `in`; each is mapped again as two partitions, the first and
+// the second half of its endpoints.  A partition with one endpoint, or whose
+// halves do not both fit a partition, stays whole.  `out` receives the
+// design with every split partition replaced by its halves (re-sorted by
+// decreasing boomerang stages); split[k] = 1 when partition k was split.
+// Throws std::runtime_error on a malformed list (a partition named twice).
+void split_partitions(const Aig &g, const Compiled &in, const uint32_t *parts,
+                      size_t n_parts, Compiled &out, std::vector<uint8_t> &split);
+
 // Block assignment ("masonry", src/flatten.rs:1207-1226) with per-partition
 // costs (nullptr = the reference heuristic stages+3) and flatten into the
 // script format.  costs is indexed [major stage][partition].

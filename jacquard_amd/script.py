@@ -62,6 +62,20 @@ class Compiled:
             raise RuntimeError("merge failed: " + N.host_error())
         return Compiled._wrap(self.netlist, h, self.log), ok[:len(p)].astype(bool)
 
+    def split(self, parts):
+        """A new Compiled with each (major, i) partition mapped again as two
+        partitions over the halves of its endpoints (a partition with one
+        endpoint, or whose halves do not both fit, stays whole).  Returns
+        (compiled, split bool[len(parts)]); partition indices of the result
+        follow its own order."""
+        p = np.ascontiguousarray(np.asarray(parts, np.uint32).reshape(-1, 2))
+        ok = np.zeros(max(len(p), 1), np.uint8)
+        h = N.host().jqh_compiled_split(self.netlist.handle, self._h, p.ctypes.data,
+                                        len(p), ok.ctypes.data)
+        if not h:
+            raise RuntimeError("split failed: " + N.host_error())
+        return Compiled._wrap(self.netlist, h, self.log), ok[:len(p)].astype(bool)
+
     def endpoint_counts(self):
         """Endpoints per partition, [major][partition]."""
         L = N.host()

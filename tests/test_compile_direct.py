@@ -165,3 +165,38 @@ def test_one_boomerang_stage_per_partition_vs_direct():
     st = stimulus.build_states(s, cycles, seed=8, p=0.3)
     oracle.run(s, st, cycles, sram=np.zeros(s.sram_storage_size, np.uint32))
     direct_check(s, st, cycles, 8, 0.3)
+
+
+@pytest.mark.parametrize("xprop", [False, True])
+def test_split_partitions_vs_direct(xprop):
+    """Partition cut (Compiled.split): the largest partition of every stage
+    mapped again as two halves of its endpoints; the design has more
+    partitions, passes the validator, and its values (and X) equal the
+    direct AIG simulation."""
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    nl = Netlist.random(seed=24, gates=20000, depth=30, dffs=1500, inputs=48, outputs=24,
+                        geometric_p=0.3, srams=2)
+    comp = Compiled(nl, level_split=5)
+    pps = comp.parts_per_stage()
+    eps = comp.endpoint_counts()
+    parts = [(m, int(np.argmax(e))) for m, e in enumerate(eps) if max(e) > 1]
+    assert parts
+    sc, ok = comp.split(parts)
+    assert ok.any()
+    assert sum(sc.parts_per_stage()) == sum(pps) + int(ok.sum())
+    assert [sum(c) for c in sc.endpoint_counts()] == [sum(c) for c in eps]
+    s = sc.flatten(4, xprop=xprop)
+    s.validate()
+    cycles = 50
+    m, pick = (None, None)
+    if xprop:
+        m, pick = stimulus.dff_initial_xmask(s, 0.1, seed=4)
+    st = stimulus.build_states(s, cycles, seed=7, p=0.3, initial_xmask=m)
+    sram = np.zeros(s.sram_storage_size, np.uint32)
+    sram_x = np.full(s.sram_storage_size, 0xFFFFFFFF, np.uint32) if xprop else None
+    oracle.run(s, st, cycles, sram=sram, sram_x=sram_x)
+    direct_check(s, st, cycles, 7, 0.3, xprop=xprop,
+                 dff_init_x=None if pick is None else pick.astype(np.uint8))
+    with pytest.raises(RuntimeError):
+        comp.split([(0, 0), (0, 0)])
