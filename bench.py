@@ -679,7 +679,9 @@ def forms_1m(a, torch):
                      avg_launch_s / 1e9, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "algorithmic_bytes_per_cycle": bpc, "avg_launch_ms": avg_launch_s * 1e3}}}
     out["plain_1m"]["roofline"]["frac"] = out["plain_1m"]["roofline"]["achieved"] / HBM_PEAK_GBPS
-    log(f"forms: 1M plain {plain:,.0f} cycles/s")
+    out["plain_1m"]["sync_ab"] = ab = plain_sync_ab(script, torch)
+    log(f"forms: 1M plain {plain:,.0f} cycles/s; sync A/B over {ab['cycles']} cycles: "
+        f"dataflow {ab['dataflow_cycles_per_s']:,.0f}, barrier {ab['barrier_cycles_per_s']:,.0f}")
     from jacquard_amd.script import Compiled
     comp_b = Compiled(nl, level_split=a.batched_split)
     b = bench_batched(f, comp_b, torch, steps=0, learned=False)
@@ -772,7 +774,8 @@ def forms_16m(a, torch):
     x 20,000 cycles through 2 groups x 128 blocks with the learned block
     assignment (A/B against the heuristic; no merge and no tiling sweep,
     whose flattenings of a 0.9 GB script take tens of seconds each) in one
-    graph replay, and the plain drop-in over 2 x 1,000 cycles."""
+    graph replay, and the plain drop-in (compiled at level split 4) over
+    2 x 1,000 cycles."""
     import ctypes
     from jacquard_amd import _native as N
     from jacquard_amd import batched as B
@@ -795,7 +798,10 @@ def forms_16m(a, torch):
     value = b["lane_cycles"] / b["wall"]
     res = ctypes.c_int(0)
     N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
-    script = comp.flatten(min(max(comp.parts_per_stage()), res.value))
+    # the plain drop-in at its own best level split: 4 levels per major
+    # stage (32 stages) ran 2,870 cycles/s against 2,560 at split 8
+    comp_p = Compiled(nl, level_split=4)
+    script = comp_p.flatten(min(max(comp_p.parts_per_stage()), res.value))
     p = bench_plain(f, script, torch)
     plain = p["cycles"] / p["wall"]
     log(f"forms: 16M batched {value:,.0f} lane-cycles/s, plain {plain:,.0f} cycles/s")
@@ -809,6 +815,7 @@ def forms_16m(a, torch):
         "partitions": b["script"].num_parts, "script_bytes": int(b["script"].blocks_data.nbytes),
         "host_generate_compile_s": t_host, "verified_cycles": b["verified_cycles"],
         "plain_lane_cycles_per_s": plain, "plain_blocks": script.num_blocks,
+        "plain_level_split": 4, "plain_major_stages": script.num_major_stages,
         "plain_verified_cycles": p.get("verified_cycles"), "vs_plain": value / plain}}
 
 
@@ -1055,16 +1062,68 @@ def barrier_us_per_stage(script, torch, cycles=2000):
                                    st.data_ptr(), None, None, 0, 0,
                                    ctypes.c_void_p(stream.cuda_stream)))
 
-    launch()
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s.record(stream)
-    for _ in range(3):
+    # value-only launches of an empty script would wait for nobody under
+    # dataflow synchronisation: time the grid barrier itself
+    prev = L.jq_scan_sync_mode(0)
+    try:
         launch()
-    e.record(stream)
-    e.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        for _ in range(3):
+            launch()
+        e.record(stream)
+        e.synchronize()
+    finally:
+        L.jq_scan_sync_mode(prev)
     if sim.device_error():
         raise RuntimeError("device barrier timeout during the barrier measurement")
     return s.elapsed_time(e) / 3 / (cycles * M) * 1e3
+
+
+def plain_sync_ab(script, torch, cycles=4000, seed=3, reps=2):
+    """cycles/s of the plain drop-in under dataflow synchronisation (the
+    default) and under grid barriers, alternating launches of the same
+    stimulus; the two launches' snapshots must be bit-identical."""
+    import ctypes
+    from jacquard_amd import _native as N
+    from jacquard_amd import sim, stimulus
+    L = N.hip()
+    ss = script.effective_state_size()
+    st0 = sim.to_device(stimulus.build_states(script, cycles, seed=seed, p=0.1))
+    d_bs = sim.to_device(np.ascontiguousarray(script.blocks_start, np.uint64))
+    d_bd = sim.to_device(np.ascontiguousarray(script.blocks_data, np.uint32))
+    stream = torch.cuda.current_stream()
+    rates = {1: [], 0: []}
+    finals = {}
+    prev = L.jq_scan_sync_mode(-1)
+    try:
+        for _ in range(reps):
+            for mode in (1, 0):
+                L.jq_scan_sync_mode(mode)
+                st = st0.clone()
+                sram = torch.zeros(max(script.sram_storage_size, 1), dtype=torch.int32,
+                                   device="cuda")
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                N.check(L.jq_simulate_scan(
+                    script.num_blocks, script.num_major_stages, d_bs.data_ptr(),
+                    d_bd.data_ptr(), sram.data_ptr(), sram.data_ptr(), cycles, ss,
+                    st.data_ptr(), None, None, 0, 0, ctypes.c_void_p(stream.cuda_stream)))
+                e1.record(stream)
+                e1.synchronize()
+                rates[mode].append(cycles / (e0.elapsed_time(e1) / 1e3))
+                finals[mode] = st
+    finally:
+        L.jq_scan_sync_mode(prev)
+    if sim.device_error():
+        raise RuntimeError("device wait timed out during the sync A/B")
+    same = bool(torch.equal(finals[0], finals[1]))
+    if not same:
+        raise RuntimeError("dataflow and barrier schedules disagree")
+    return {"dataflow_cycles_per_s": float(np.mean(rates[1])),
+            "barrier_cycles_per_s": float(np.mean(rates[0])),
+            "cycles": cycles, "launches_each": reps, "identical": same}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "scan_kernel_1M_gates_hbm_traffic.json")
@@ -1203,6 +1262,9 @@ def main():
     stage_us = avg_launch_s * 1e6 / (a.cycles_per_step * script.num_major_stages)
     log(f"barrier: {bar_us:.2f} us per major stage with empty blocks, "
         f"{stage_us:.2f} us per major stage in the run")
+    ab = plain_sync_ab(script, torch)
+    log(f"sync A/B over {ab['cycles']} cycles: dataflow {ab['dataflow_cycles_per_s']:,.0f}, "
+        f"barrier {ab['barrier_cycles_per_s']:,.0f} cycles/s")
     if not a.no_batched:
         from jacquard_amd.script import Compiled
         comp_b = comp if a.batched_split == a.split else Compiled(nl, level_split=a.batched_split)
@@ -1294,6 +1356,7 @@ def main():
             "us_per_major_stage_run": stage_us,
             "blocks": script.num_blocks,
         },
+        "sync_ab": ab,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -218,6 +218,17 @@ int jq_max_resident_blocks(int kind, int *out_blocks);
  * state; clear only while no launch runs on the device. */
 int jq_device_error_flag(u32 *out, int clear);
 
+/* Synchronisation of value-only scan launches (no counterpart in the
+ * reference, whose kernel always takes a grid barrier after each major
+ * stage, csrc/kernel_v1_impl.cuh:565-626).  mode 1 (the default): every
+ * launch without timing first derives from the script which blocks each
+ * (block, major stage) reads, and a block then waits only for those
+ * blocks' progress; scripts with more than 256 blocks or 120 major stages
+ * still use grid barriers.  mode 0: grid barriers everywhere.  mode < 0
+ * only queries.  Returns the previous mode.  Results are identical in both
+ * modes; set it only while no scan launch is being issued. */
+int jq_scan_sync_mode(int mode);
+
 /* ---- Output change extraction (no counterpart in the reference) ---------
  * Replaces the host-side scan of every snapshot in write_output_vcd_impl
  * (src/sim/vcd_io.rs:758-820): for each of num_snapshots emitted snapshots

@@ -132,6 +132,7 @@ HIP_SYMBOLS = [
     "jq_cosim_flash_state_bytes",
     "jq_cosim_edges",
     "jq_device_error_flag",
+    "jq_scan_sync_mode",
     "jq_mlp_forward",
     "jq_mlp_train_step",
     "jq_mlp_num_params",
@@ -205,6 +206,7 @@ def hip():
                                                      ctypes.c_float, c_vp, c_vp]),
         "jq_last_error": (c_cp, []),
         "jq_device_error_flag": (ctypes.c_int, [P(c_u32), ctypes.c_int]),
+        "jq_scan_sync_mode": (ctypes.c_int, [ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         try:

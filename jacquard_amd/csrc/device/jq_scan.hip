@@ -2,8 +2,9 @@
 //
 // One workgroup per script block, persistent over all cycles and major
 // stages, a grid barrier after every major stage (the contract of
-// csrc/kernel_v1_impl.cuh:565-626).  Inside a partition the work is mapped
-// to CDNA4 as follows:
+// csrc/kernel_v1_impl.cuh:565-626) or, in value-only launches, a wait for
+// just the blocks whose words the next stage reads.  Inside a partition the
+// work is mapped to CDNA4 as follows:
 //   - script sections are streamed with 16-byte per-thread loads (each
 //     section is one fully coalesced 4 KiB request), the next boomerang
 //     stage's permutation words are in flight while the current tree runs,
@@ -21,6 +22,7 @@
 // setup/hold rules, with the writeout arrival reset at every partition.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -33,6 +35,12 @@ namespace jqd {
 
 __device__ u32 g_barrier[kCohSlots * kCohSlotWords];
 __device__ u32 g_error;
+// progress words of the dataflow-synchronised launches: per barrier slot,
+// 256 words one 128-byte line apart (packed into 8 lines, every block's
+// polls and the producers' stores met on the same lines, and the 1M-gate
+// launch ran 20% slower than with grid barriers)
+constexpr u32 kProgStride = 32;
+__device__ u32 g_prog[kCohSlots * 256 * kProgStride];
 
 namespace {
 
@@ -59,6 +67,12 @@ struct ScanArgs {
   u32 ev_cap;
   i32 arrival_state_offset;
   u32 *bar;
+  // dataflow synchronisation (value-only launches): the producer table of
+  // build_flow_tables, the flag its builders raise when a script does not
+  // fit it, and the blocks' progress words; dep == null: grid barriers
+  const u32 *dep;
+  const u32 *dep_bad;
+  u32 *prog;
   // profiling
   const u32 *part_index;
   u32 num_parts;
@@ -529,6 +543,158 @@ __device__ void run_partition(const ScanArgs &A, const u32 *__restrict__ bd,
     __syncthreads();
 }
 
+// ---- dataflow synchronisation of value-only launches
+//
+// Partitions read snapshot words that other blocks wrote in an earlier
+// major stage of the same cycle (`out`, bit 31 of a global-read entry) or
+// in any stage of the previous cycle (`in`); every cycle has its own
+// snapshot and SRAMs belong to one partition, so those reads are the only
+// dependencies between blocks.  Instead of a grid barrier after every major
+// stage, each block publishes its progress (c * M + m + 1 once it has
+// finished major stage m of cycle c and its stores have completed) and,
+// before a stage, waits only for the blocks it reads from, each up to the
+// stage that wrote the words.  The producer table holds, per (block, major
+// stage), one u32 per lane of wave 0: byte j of lane L is the producer
+// block L + 64 j's offset d (its progress must reach c * M + d), or 0x80
+// when the stage reads nothing of that block.  Results are those of the
+// barrier schedule: every read still sees exactly the writes it saw there.
+constexpr u32 kFlowMaxBlocks = 256;
+constexpr u32 kFlowMaxStages = 120;
+constexpr u32 kFlowNone = 0xFFFFFFFFu;
+constexpr u32 kFlowSpins = 1u << 22;
+
+struct FlowBuild {
+  const usize *blocks_start;
+  const u32 *bd;
+  u32 nb, M;
+  usize state_size;
+  u32 *owner;  // state_size words: (block << 16 | major stage) of the writer
+  u32 *dep;    // nb * M * 64 words
+  u32 *bad;    // nonzero: the launch falls back to grid barriers
+};
+
+__global__ __launch_bounds__(256) void flow_fill_kernel(u32 *p, usize n, u32 v) {
+  for (usize i = (usize)blockIdx.x * 256u + threadIdx.x; i < n; i += (usize)gridDim.x * 256u)
+    p[i] = v;
+}
+
+// Workgroup (m, b) records the writer of every writeout word of block b's
+// partitions in major stage m; a word written twice or out of range makes
+// the table unusable.
+__global__ __launch_bounds__(256) void flow_owner_kernel(FlowBuild F) {
+  const u32 m = blockIdx.x / F.nb, b = blockIdx.x % F.nb;
+  size_t off = F.blocks_start[(size_t)m * F.nb + b];
+  for (;;) {
+    const u32 *s = F.bd + off;
+    const u32 ns = s[0], is_last = s[1], nio = s[2], io = s[3], rounds = s[6];
+    if (ns == 0) break;
+    if (threadIdx.x < nio) {
+      const usize w = (usize)io + threadIdx.x;
+      if (w >= F.state_size)
+        atomicOr(F.bad, 1u);
+      else if (atomicExch(&F.owner[w], (b << 16) | m) != kFlowNone)
+        atomicOr(F.bad, 2u);
+    }
+    if (is_last) break;
+    off += 256 + (size_t)512 * rounds + (size_t)5120 * (ns + 2);
+  }
+}
+
+// Workgroup (m, b) reduces the global-read entries of block b's partitions
+// in major stage m to the latest progress it needs of every other block.
+__global__ __launch_bounds__(256) void flow_dep_kernel(FlowBuild F) {
+  __shared__ u32 r[kFlowMaxBlocks];
+  const u32 m = blockIdx.x / F.nb, b = blockIdx.x % F.nb;
+  const u32 t = threadIdx.x;
+  r[t] = 0;
+  __syncthreads();
+  size_t off = F.blocks_start[(size_t)m * F.nb + b];
+  for (;;) {
+    const u32 *s = F.bd + off;
+    const u32 ns = s[0], is_last = s[1], rounds = s[6];
+    if (ns == 0) break;
+    const uint2 *gr = reinterpret_cast<const uint2 *>(s + 256);
+    for (u32 k = 0; k < rounds; ++k) {
+      const uint2 e = gr[256 * k + t];
+      if (!e.y) continue;
+      const u32 w = e.x & 0x7fffffffu;
+      const bool from_out = (e.x >> 31) != 0;
+      if (w >= F.state_size) {
+        atomicOr(F.bad, 4u);
+        continue;
+      }
+      const u32 o = F.owner[w];
+      if (o == kFlowNone) continue;  // a primary input: written by the host
+      const u32 q = o >> 16, sq = o & 0xffffu;
+      if (q == b) continue;  // the block's own earlier work
+      if (from_out && sq >= m) {  // not written before this stage
+        atomicOr(F.bad, 8u);
+        continue;
+      }
+      const int d = from_out ? (int)sq + 1 : (int)sq + 1 - (int)F.M;
+      atomicMax(&r[q], (u32)(d + 128));
+    }
+    if (is_last) break;
+    off += 256 + (size_t)512 * rounds + (size_t)5120 * (ns + 2);
+  }
+  __syncthreads();
+  if (t < 64) {
+    u32 word = 0;
+#pragma unroll
+    for (u32 j = 0; j < 4; ++j) {
+      const u32 v = r[t + 64 * j];
+      word |= (v ? ((v - 128u) & 0xffu) : 0x80u) << (8 * j);
+    }
+    F.dep[((size_t)b * F.M + m) * 64 + t] = word;
+  }
+}
+
+// Publishes `done` as block b's progress once every wave has waited for its
+// stores, then wave 0 polls the progress of the producers in `row` until
+// each reaches base + its offset.  The wait is bounded: after kFlowSpins
+// polls it raises `err` and the block stops waiting for the rest of the
+// launch, so the grid always drains.
+__device__ __forceinline__ void flow_publish_wait(u32 *prog, u32 b, u32 done, u32 row,
+                                                  u32 base, bool &failed, u32 *err) {
+  __syncthreads();
+  if (threadIdx.x == 0) st_coherent(&prog[b * kProgStride], done);
+  if (threadIdx.x < 64 && !failed && row != 0x80808080u) {
+    const u32 lane = threadIdx.x;
+    u32 tg[4];
+    bool need[4];
+#pragma unroll
+    for (u32 j = 0; j < 4; ++j) {
+      const u32 d = (row >> (8 * j)) & 0xffu;
+      need[j] = d != 0x80u;
+      tg[j] = base + (u32)(int)(signed char)d;
+    }
+    u32 spins = 0;
+    for (;;) {
+      u32 p[4];
+#pragma unroll
+      for (u32 j = 0; j < 4; ++j) p[j] = need[j] ? ld_coherent(&prog[(lane + 64 * j) * kProgStride]) : 0u;
+      bool pend = false;
+#pragma unroll
+      for (u32 j = 0; j < 4; ++j)
+        if (need[j]) {
+          if ((int)(p[j] - tg[j]) >= 0)
+            need[j] = false;
+          else
+            pend = true;
+        }
+      if (!__any(pend)) break;
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kFlowSpins) {
+        if (lane == 0) atomicOr(err, 1u);
+        failed = true;
+        break;
+      }
+    }
+  }
+  asm volatile("" ::: "memory");
+  __syncthreads();
+}
+
 template <bool TIMING, bool PROFILE>
 __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
   if (A.timing) {
@@ -552,6 +718,14 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
   // the block's partitions in execution order: cycle, major stage, then
   // position in the stage's section of blocks_data (kernel_v1_impl.cuh:
   // 565-626); h / P always describe the next partition to run
+  // value-only launches whose producer table was built synchronise by
+  // dataflow; the flag is the same word for every block
+  const bool flow = !TIMING && !PROFILE && A.dep != nullptr && *A.dep_bad == 0;
+  bool flow_failed = false;
+  if (flow) {
+    if (t == 0) st_coherent(&A.prog[b * kProgStride], 0u);
+    grid_sync_coherent(A.bar, nb, b, gen, &g_error);
+  }
   PartHdr h = M ? load_hdr(bd, A.blocks_start[b]) : no_hdr();
   PartPre P;
   if (h.num_stages) load_pre(P, bd, h, A.states);
@@ -562,6 +736,10 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
       u32 part = PROFILE ? A.part_index[(size_t)m * nb + b] : 0u;
       const bool wrap = m + 1 == M;
       const u32 nc = wrap ? c + 1 : c, nm = wrap ? 0u : m + 1;
+      // the next stage's producer row is read a stage ahead of its use
+      u32 row = 0x80808080u;
+      if (flow && t < 64 && nc < A.num_cycles)
+        row = A.dep[((size_t)b * M + nm) * 64 + t];
       for (;;) {
         const bool last = h.num_stages == 0 || h.is_last != 0;
         // the successor's header is requested before this partition runs
@@ -586,7 +764,11 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
         if (hn.num_stages) load_pre(P, bd, hn, in);
         h = hn;
       }
-      grid_arrive_wait(A.bar, nb, b, gen, &g_error);
+      if (flow)
+        flow_publish_wait(A.prog, b, c * M + m + 1, row, nc * M, flow_failed,
+                          &g_error);
+      else
+        grid_arrive_wait(A.bar, nb, b, gen, &g_error);
     }
   }
 }
@@ -678,6 +860,9 @@ int set_error(const char *msg) {
 
 namespace {
 
+// 1: value-only launches synchronise by dataflow; 0: by grid barriers.
+int g_scan_sync = 1;
+
 template <bool TIMING, bool PROFILE>
 int launch_scan(ScanArgs &A, hipStream_t stream) {
   static_assert(sizeof(EvHdr) == 16 && sizeof(Ev) == 32, "event layout");
@@ -685,6 +870,12 @@ int launch_scan(ScanArgs &A, hipStream_t stream) {
   if (hipGetSymbolAddress((void **)&bar_base, HIP_SYMBOL(g_barrier)) != hipSuccess)
     return fail("hipGetSymbolAddress(g_barrier) failed");
   A.bar = bar_base + kCohSlotWords * (__atomic_fetch_add(&g_slot, 1u, __ATOMIC_RELAXED) % kCohSlots);
+  if (A.dep) {
+    u32 *pb = nullptr;
+    if (hipGetSymbolAddress((void **)&pb, HIP_SYMBOL(g_prog)) != hipSuccess)
+      return fail("hipGetSymbolAddress(g_prog) failed");
+    A.prog = pb + (size_t)256 * kProgStride * ((A.bar - bar_base) / kCohSlotWords);
+  }
   void *args[] = {&A};
   hipError_t e = hipLaunchCooperativeKernel(
       reinterpret_cast<const void *>(&scan_kernel<TIMING, PROFILE>),
@@ -692,6 +883,51 @@ int launch_scan(ScanArgs &A, hipStream_t stream) {
   if (e != hipSuccess)
     return fail(std::string("scan launch failed: ") + hipGetErrorString(e));
   return 0;
+}
+
+// Builds the producer table of a value-only launch on `stream` ahead of it
+// and returns its stream-ordered allocation (freed after the launch), or
+// null when the launch runs with grid barriers: the mode is off, the
+// script is outside the table's limits, or the allocation failed.  A
+// script that the builders find unusable (a writeout word written twice or
+// out of range, a same-cycle read of a word not written in an earlier
+// stage) raises the flag that the kernel reads, and the launch then uses
+// grid barriers as well.
+void *build_flow_tables(ScanArgs &A, hipStream_t stream) {
+  A.dep = nullptr;
+  A.dep_bad = nullptr;
+  const u32 nb = A.num_blocks, M = A.num_major_stages;
+  if (!__atomic_load_n(&g_scan_sync, __ATOMIC_RELAXED) || nb > kFlowMaxBlocks || M == 0 ||
+      M > kFlowMaxStages || A.state_size == 0 || A.state_size > (usize(1) << 28))
+    return nullptr;
+  const size_t own_words = ((size_t)A.state_size + 63) & ~(size_t)63;
+  const size_t dep_words = (size_t)nb * M * 64;
+  void *buf = nullptr;
+  if (hipMallocAsync(&buf, 4 * (own_words + dep_words + 64), stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  FlowBuild F{};
+  F.blocks_start = A.blocks_start;
+  F.bd = A.blocks_data;
+  F.nb = nb;
+  F.M = M;
+  F.state_size = A.state_size;
+  F.owner = static_cast<u32 *>(buf);
+  F.dep = F.owner + own_words;
+  F.bad = F.dep + dep_words;
+  const u32 fill_blocks = (u32)std::min<size_t>(1024, (own_words + 255) / 256);
+  flow_fill_kernel<<<fill_blocks, 256, 0, stream>>>(F.owner, A.state_size, kFlowNone);
+  flow_fill_kernel<<<1, 256, 0, stream>>>(F.bad, 1, 0u);
+  flow_owner_kernel<<<nb * M, 256, 0, stream>>>(F);
+  flow_dep_kernel<<<nb * M, 256, 0, stream>>>(F);
+  if (hipGetLastError() != hipSuccess) {
+    (void)hipFreeAsync(buf, stream);
+    return nullptr;
+  }
+  A.dep = F.dep;
+  A.dep_bad = F.bad;
+  return buf;
 }
 
 int scan_common(usize num_blocks, usize num_major_stages, const usize *blocks_start,
@@ -727,7 +963,10 @@ int scan_common(usize num_blocks, usize num_major_stages, const usize *blocks_st
   const bool timing = A.timing != nullptr || arrival_state_offset != 0;
   if (samples) return launch_scan<false, true>(A, stream);
   if (timing) return launch_scan<true, false>(A, stream);
-  return launch_scan<false, false>(A, stream);
+  void *flow_buf = build_flow_tables(A, stream);
+  const int rc = launch_scan<false, false>(A, stream);
+  if (flow_buf) (void)hipFreeAsync(flow_buf, stream);
+  return rc;
 }
 
 void die_on(int rc) {
@@ -812,6 +1051,11 @@ extern "C" int jq_max_resident_blocks(int kind, int *out_blocks) {
   }
   *out_blocks = per * cus;
   return 0;
+}
+
+extern "C" int jq_scan_sync_mode(int mode) {
+  if (mode < 0) return __atomic_load_n(&g_scan_sync, __ATOMIC_RELAXED);
+  return __atomic_exchange_n(&g_scan_sync, mode ? 1 : 0, __ATOMIC_RELAXED);
 }
 
 extern "C" int jq_device_error_flag(u32 *out, int clear) {

@@ -17,6 +17,21 @@ def _say(*a):
     print(*a, flush=True)
 
 
+class _sync_mode:
+    """Sets the value-only scan launches' synchronisation (1: dataflow, the
+    default; 0: grid barriers) for the duration of a with-block."""
+
+    def __init__(self, mode):
+        from jacquard_amd import _native as N
+        self.L, self.mode = N.hip(), mode
+
+    def __enter__(self):
+        self.prev = self.L.jq_scan_sync_mode(self.mode)
+
+    def __exit__(self, *exc):
+        self.L.jq_scan_sync_mode(self.prev)
+
+
 @pytest.mark.parametrize("num_blocks,split", [(1, 0), (8, 0), (16, 6), (64, 4)])
 def test_config1_2state(num_blocks, split):
     nl, comp = design("c1", level_split=split, **CONFIG1)
@@ -32,9 +47,12 @@ def test_config1_2state(num_blocks, split):
 
 
 @pytest.mark.parametrize("num_blocks", [128, 200])
-def test_many_blocks_hierarchical_barrier(num_blocks):
+@pytest.mark.parametrize("sync", [0, 1])
+def test_many_blocks_hierarchical_barrier(num_blocks, sync):
     """At 128 workgroups or more the grid barrier counts arrivals through
-    eight sub-counters; most blocks here are empty and still take part."""
+    eight sub-counters; most blocks here are empty and still take part
+    (sync 0).  With dataflow synchronisation (sync 1) the empty blocks
+    publish progress that nobody waits for."""
     from jacquard_amd.netlist import Netlist
     from jacquard_amd.script import Compiled
     nl = Netlist.random(seed=5, gates=20000, depth=40, dffs=1024, inputs=64, outputs=32,
@@ -46,7 +64,8 @@ def test_many_blocks_hierarchical_barrier(num_blocks):
     st = stimulus.build_states(s, cycles, seed=21, p=0.3)
     ref = st.copy()
     oracle.run(s, ref, cycles)
-    got = sim.run_scan(s, st.copy(), cycles)["states"]
+    with _sync_mode(sync):
+        got = sim.run_scan(s, st.copy(), cycles)["states"]
     assert np.array_equal(got, ref)
 
 
@@ -92,6 +111,36 @@ def test_xprop(frac):
     out = sim.run_scan(s, st.copy(), cycles)
     assert np.array_equal(out["states"], ref)
     assert np.array_equal(out["sram_x"][:s.sram_storage_size], sram_x)
+
+
+@pytest.mark.parametrize("xprop", [False, True])
+def test_dataflow_and_barrier_sync_agree(xprop):
+    """Value-only launches wait only for the blocks whose words the next
+    major stage reads (each up to the stage that wrote them) instead of
+    taking a grid barrier.  Both schedules must give the oracle's bits:
+    every snapshot, the SRAM contents and their X shadow."""
+    from jacquard_amd import _native as N
+    assert N.hip().jq_scan_sync_mode(-1) == 1, "dataflow is the default"
+    nl, comp = design("sync", level_split=4, seed=9, gates=40000, depth=48,
+                      dffs=3000, inputs=64, outputs=32, geometric_p=0.3, srams=3)
+    s = comp.flatten(48, xprop=xprop)
+    cycles = 100
+    _say("sync modes", xprop, s.num_blocks, s.num_major_stages, s.num_parts)
+    m = None
+    if xprop:
+        m, _pick = stimulus.dff_initial_xmask(s, 0.1, seed=8)
+    st = stimulus.build_states(s, cycles, seed=15, p=0.3, initial_xmask=m)
+    ref = st.copy()
+    sram = np.zeros(s.sram_storage_size, np.uint32)
+    sram_x = np.full(s.sram_storage_size, 0xFFFFFFFF, np.uint32)
+    oracle.run(s, ref, cycles, sram=sram, sram_x=sram_x if xprop else None)
+    for mode in (1, 0):
+        with _sync_mode(mode):
+            out = sim.run_scan(s, st.copy(), cycles)
+        assert np.array_equal(out["states"], ref), mode
+        assert np.array_equal(out["sram"][:s.sram_storage_size], sram), mode
+        if xprop:
+            assert np.array_equal(out["sram_x"][:s.sram_storage_size], sram_x), mode
 
 
 def test_timing_events():
