@@ -223,8 +223,12 @@ int jq_device_error_flag(u32 *out, int clear);
  * stage, csrc/kernel_v1_impl.cuh:565-626).  mode 1 (the default): every
  * launch without timing first derives from the script which blocks each
  * (block, major stage) reads, and a block then waits only for those
- * blocks' progress; scripts with more than 256 blocks or 120 major stages
- * still use grid barriers.  mode 0: grid barriers everywhere.  mode < 0
+ * blocks' progress; launches of fewer than 8 cycles and scripts with more
+ * than 256 blocks or 120 major stages still use grid barriers.  The
+ * derived table is the one allocation these launches make: 4 bytes per
+ * state word plus 256 bytes per (block, major stage), taken with
+ * hipMallocAsync on the launch's stream and released with hipFreeAsync
+ * after it (if the allocation fails, the launch uses grid barriers).  mode 0: grid barriers everywhere.  mode < 0
  * only queries.  Returns the previous mode.  Results are identical in both
  * modes; set it only while no scan launch is being issued. */
 int jq_scan_sync_mode(int mode);

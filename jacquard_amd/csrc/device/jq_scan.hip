@@ -562,6 +562,7 @@ constexpr u32 kFlowMaxBlocks = 256;
 constexpr u32 kFlowMaxStages = 120;
 constexpr u32 kFlowNone = 0xFFFFFFFFu;
 constexpr u32 kFlowSpins = 1u << 22;
+constexpr u32 kFlowMinCycles = 8;
 
 struct FlowBuild {
   const usize *blocks_start;
@@ -888,7 +889,9 @@ int launch_scan(ScanArgs &A, hipStream_t stream) {
 // Builds the producer table of a value-only launch on `stream` ahead of it
 // and returns its stream-ordered allocation (freed after the launch), or
 // null when the launch runs with grid barriers: the mode is off, the
-// script is outside the table's limits, or the allocation failed.  A
+// script is outside the table's limits, the launch is shorter than
+// kFlowMinCycles (the table's ~30 us build is then comparable to the
+// barriers it saves), or the allocation failed.  A
 // script that the builders find unusable (a writeout word written twice or
 // out of range, a same-cycle read of a word not written in an earlier
 // stage) raises the flag that the kernel reads, and the launch then uses
@@ -898,7 +901,8 @@ void *build_flow_tables(ScanArgs &A, hipStream_t stream) {
   A.dep_bad = nullptr;
   const u32 nb = A.num_blocks, M = A.num_major_stages;
   if (!__atomic_load_n(&g_scan_sync, __ATOMIC_RELAXED) || nb > kFlowMaxBlocks || M == 0 ||
-      M > kFlowMaxStages || A.state_size == 0 || A.state_size > (usize(1) << 28))
+      M > kFlowMaxStages || A.num_cycles < kFlowMinCycles || A.state_size == 0 ||
+      A.state_size > (usize(1) << 28))
     return nullptr;
   const size_t own_words = ((size_t)A.state_size + 63) & ~(size_t)63;
   const size_t dep_words = (size_t)nb * M * 64;
