@@ -143,6 +143,39 @@ def test_dataflow_and_barrier_sync_agree(xprop):
             assert np.array_equal(out["sram_x"][:s.sram_storage_size], sram_x), mode
 
 
+@pytest.mark.parametrize("cycles", [1, 7, 8, 9])
+def test_launch_lengths_around_the_dataflow_threshold(cycles):
+    """Launches shorter than 8 cycles keep grid barriers; from 8 cycles on
+    the producer table is built.  Both sides of the threshold, on a
+    multi-block script with SRAMs, give the oracle's snapshots and SRAMs."""
+    nl, comp = design("sync", level_split=4, seed=9, gates=40000, depth=48,
+                      dffs=3000, inputs=64, outputs=32, geometric_p=0.3, srams=3)
+    s = comp.flatten(48)
+    st = stimulus.build_states(s, cycles, seed=16, p=0.3)
+    ref = st.copy()
+    sram = np.zeros(s.sram_storage_size, np.uint32)
+    oracle.run(s, ref, cycles, sram=sram)
+    out = sim.run_scan(s, st.copy(), cycles)
+    assert np.array_equal(out["states"], ref)
+    assert np.array_equal(out["sram"][:s.sram_storage_size], sram)
+
+
+def test_more_major_stages_than_the_dataflow_table_holds():
+    """One logic level per major stage on a depth-140 netlist gives more
+    than 120 major stages: the launch keeps grid barriers and stays exact."""
+    nl, comp = design("deep", level_split=1, seed=12, gates=6000, depth=140,
+                      dffs=256, inputs=32, outputs=16, geometric_p=0.3)
+    s = comp.flatten(8)
+    assert s.num_major_stages > 120
+    cycles = 20
+    _say("deep", s.num_major_stages, s.num_parts)
+    st = stimulus.build_states(s, cycles, seed=17, p=0.3)
+    ref = st.copy()
+    oracle.run(s, ref, cycles)
+    got = sim.run_scan(s, st.copy(), cycles)["states"]
+    assert np.array_equal(got, ref)
+
+
 def test_timing_events():
     nl, comp = design("tm", level_split=6, seed=6, gates=8000, depth=30,
                       dffs=800, inputs=32, outputs=16, geometric_p=0.3)
