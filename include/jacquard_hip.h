@@ -218,6 +218,19 @@ int jq_max_resident_blocks(int kind, int *out_blocks);
  * state; clear only while no launch runs on the device. */
 int jq_device_error_flag(u32 *out, int clear);
 
+/* Bound of the device waits (no counterpart in the reference, whose grid
+ * barrier waits without a bound).  Every grid barrier and dataflow wait of
+ * the scan, co-simulation and batched launches gives up after `polls`
+ * polls (about 1-2 us each; default 2^23, several seconds): it raises the
+ * flag of jq_device_error_flag and marks its launch aborted; the launch's
+ * other waits notice within 256 polls and every workgroup leaves at its
+ * next cycle, so a launch whose workgroups are not all resident ends soon
+ * after one time-out.  polls > 0 sets the bound (capped at 2^32 - 1; set it
+ * only while no launch runs); 0 only queries.  Returns the previous bound,
+ * or 0 if the device could not be reached.  For hosts that want a shorter
+ * bound and for tests of the time-out path. */
+usize jq_wait_limit(usize polls);
+
 /* Synchronisation of value-only scan launches (no counterpart in the
  * reference, whose kernel always takes a grid barrier after each major
  * stage, csrc/kernel_v1_impl.cuh:565-626).  mode 1 (the default): every

@@ -133,6 +133,7 @@ HIP_SYMBOLS = [
     "jq_cosim_edges",
     "jq_device_error_flag",
     "jq_scan_sync_mode",
+    "jq_wait_limit",
     "jq_mlp_forward",
     "jq_mlp_train_step",
     "jq_mlp_num_params",
@@ -207,6 +208,7 @@ def hip():
         "jq_last_error": (c_cp, []),
         "jq_device_error_flag": (ctypes.c_int, [P(c_u32), ctypes.c_int]),
         "jq_scan_sync_mode": (ctypes.c_int, [ctypes.c_int]),
+        "jq_wait_limit": (c_u64, [c_u64]),
     }
     for name, (res, args) in sig.items():
         try:

@@ -39,6 +39,7 @@ namespace jqd {
 
 constexpr int kBatchedSlots = 256;  // lane groups in flight
 __device__ u32 gb_barrier[kBatchedSlots * kCohSlotWords];
+__device__ u32 gb_wait_polls = kDefaultWaitPolls;  // bound of every wait here
 __device__ u32 gb_error;
 
 namespace {
@@ -46,8 +47,11 @@ namespace {
 constexpr int kLayerWords = 8192;
 constexpr int kSecWords = 5120;  // 4096 permutation words + 256 x 4 flags
 // LDS: the layer, the writeout area, two script-section buffers and the
-// commit's old writeout words (136 KB of the 160 KB)
-constexpr int kLdsWords = 3 * kLayerWords + 2 * kSecWords;
+// commit's old writeout words (136 KB of the 160 KB), then 4 words whose
+// first is the launch's abort flag (kept in the dynamic region: a static
+// __shared__ word would move the region's start off its 256-byte boundary)
+constexpr int kLdsWords = 3 * kLayerWords + 2 * kSecWords + 4;
+constexpr int kAbortLds = 3 * kLayerWords + 2 * kSecWords;
 constexpr u32 kNoSource = 0xFFFFFFFFu;
 
 struct BArgs {
@@ -432,6 +436,8 @@ __device__ __forceinline__ void run_block_lanes(const BArgs &A, const u32 *__res
 
 __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
   extern __shared__ u32 lds[];
+  u32 &s_aborted = lds[kAbortLds];  // set by a group wait when the launch aborts
+  if (threadIdx.x == 0) s_aborted = 0;
   const u32 g = blockIdx.x / A.W, b = blockIdx.x % A.W;
   u32 *bar = A.bar + kCohSlotWords * g;
   u32 gen = read_gen(bar);
@@ -467,6 +473,9 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
   __syncthreads();
   u32 c = c0;
   for (;; ++c) {
+    // an aborted launch (a bounded wait gave up; the waits that follow end
+    // within 256 polls each) leaves at the next cycle
+    if (s_aborted) break;
     u32 *in = (c & 1) ? st1 : st0;
     u32 *out = (c & 1) ? st0 : st1;
     // every group finishes within obs_cap cycles (queue bound); a launch
@@ -498,7 +507,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
           st_coherent(slots + l, kNoLane);
         }
       }
-      grid_sync_coherent(bar, A.W, b, gen, &gb_error);
+      grid_sync_coherent(bar, A.W, b, gen, &gb_error, &gb_wait_polls, s_aborted);
       // reset the finished slots' lane bit: state of this cycle's input,
       // clock flags set, the lane's SRAM words zeroed
       for (size_t w = par; w < snap; w += npar) st_coherent(in + w, ld_coherent(in + w) & ~finmask);
@@ -520,7 +529,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
           }
         }
       }
-      grid_sync_coherent(bar, A.W, b, gen, &gb_error);
+      grid_sync_coherent(bar, A.W, b, gen, &gb_error, &gb_wait_polls, s_aborted);
       if (A.stim) {
         // packed stimulus: the new lanes' input bits of their snapshot 0
         for (u32 k0 = b * 8; k0 < A.n_in; k0 += A.W * 8) {
@@ -537,7 +546,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
             st_coherent(in + p, ld_coherent(in + p) | set);
           }
         }
-        grid_sync_coherent(bar, A.W, b, gen, &gb_error);
+        grid_sync_coherent(bar, A.W, b, gen, &gb_error, &gb_wait_polls, s_aborted);
       }
     }
     if (!__syncthreads_or(my_lane != kNoLane)) break;
@@ -585,7 +594,7 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
       run_block_lanes(A, A.bd + A.bs[k], A.gsrc + (size_t)A.gidx[k] * kLayerWords, in,
                       out, ram_group, lds, prof, h, src, A.bd + A.bs[kn],
                       A.gsrc + (size_t)A.gidx[kn] * kLayerWords);
-      grid_sync_coherent(bar, A.W, b, gen, &gb_error);
+      grid_sync_coherent(bar, A.W, b, gen, &gb_error, &gb_wait_polls, s_aborted);
     }
   }
   if (b == 0 && t == 0) A.group_cycles[g] = c;
@@ -594,6 +603,10 @@ __global__ __launch_bounds__(256) void batched_kernel(BArgs A) {
 u32 b_slot = 0;
 
 }  // namespace
+
+int batched_set_wait_limit(u32 polls) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(gb_wait_polls), &polls, 4) == hipSuccess ? 0 : -1;
+}
 
 int batched_error_flag(u32 *out, int clear) {
   u32 v = 0;

@@ -25,6 +25,9 @@ int set_error(const char *msg);
 // Host side: reads (and optionally clears) the batched kernels' error word.
 int batched_error_flag(u32 *out, int clear);
 
+// Host side: sets the batched kernels' wait bound (polls); 0 or -1.
+int batched_set_wait_limit(u32 polls);
+
 // Agent-coherent access to memory that other workgroups of the launch read
 // or write (relaxed agent-scope atomics: on gfx950 loads and stores with the
 // device-scope cache policy, which bypass the non-coherent cached copies).
@@ -50,10 +53,19 @@ __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" 
 // Arrivals and polling are relaxed agent-scope atomics: an acquire on every
 // poll would invalidate the XCD's L2 once per poll.  Waiters poll a
 // generation word on its own 128-byte line, apart from the counters' lines.
-// The wait is bounded: after ~2^27 polls it raises `err` and returns, so the
-// grid always drains.
+// The wait is bounded (a limit in device memory, set by jq_wait_limit, read
+// once per wait after the arrival).  A wait that gives up raises `err` and
+// sets the slot's abort word; every other wait of the launch reads that
+// word every 256 polls (not on entry: that would add a round trip to every
+// barrier), returns once it is set and
+// reports the abort, and the kernels then leave their cycle loops, so a
+// launch whose workgroups are not all resident drains after one time-out
+// instead of timing out at every barrier.  The word stays set until the
+// host clears the error flag, which zeroes every slot of the kind
+// (jq_device_error_flag).
 //
-// A slot is kCohSlotWords u32: the generation at [32] and two counter sets
+// A slot is kCohSlotWords u32: the generation at [32], the abort word at
+// [kAbortWord] and two counter sets
 // at 256 and 2304, used by alternate barriers, so the last arrivers' resets
 // need no wait (a set is reused two barriers later, after the resetting
 // workgroup's next arrival has waited for its stores).  Words 64..87 are
@@ -67,13 +79,37 @@ __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" 
 // this one.
 constexpr int kCohSlots = 64;
 constexpr int kCohSlotWords = 4096;
+constexpr int kAbortWord = 36;
+
+// Default bound of a wait, in polls (about 1-2 us each).
+constexpr u32 kDefaultWaitPolls = 1u << 23;
+
+// Marks the launch that owns `bar` aborted and raises `err`.
+__device__ __forceinline__ void abort_launch(u32 *bar, u32 *err) {
+  atomicOr(err, 1u);
+  st_coherent(&bar[kAbortWord], 1u);
+}
+
+// One poll's bound check of a wait that started polling with limit `lim`
+// (read from device memory once, after the arrival): true when the wait
+// has taken more than lim polls (it then aborts the launch) or, checked
+// every 256 polls, when the launch has aborted.
+__device__ __forceinline__ bool wait_gives_up(u32 *bar, u32 spins, u32 lim, u32 *err) {
+  if (spins > lim) {
+    abort_launch(bar, err);
+    return true;
+  }
+  return (spins & 255u) == 0 && ld_coherent(&bar[kAbortWord]) != 0;
+}
 
 // The arrival and wait of grid_sync_coherent, for callers that have already
 // waited for their stores (wait_vmem) in every wave and may have loads in
 // flight that the barrier must not wait for.  `member` is the workgroup's
-// index among the n (0..n-1).
+// index among the n (0..n-1).  When the launch has aborted (this wait or
+// another gave up), thread 0 sets the caller's LDS word `aborted` to 1
+// before the closing workgroup barrier; it is never cleared here.
 __device__ __forceinline__ void grid_arrive_wait(u32 *bar, u32 n, u32 member, u32 &gen,
-                                                 u32 *err) {
+                                                 u32 *err, const u32 *limit, u32 &aborted) {
   __syncthreads();
   if (threadIdx.x == 0) {
     u32 *set = bar + 256 + ((gen + 1u) & 1u) * 2048;
@@ -99,12 +135,13 @@ __device__ __forceinline__ void grid_arrive_wait(u32 *bar, u32 n, u32 member, u3
       __hip_atomic_store(&set[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&bar[32], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
+      const u32 lim = *limit;
       u32 spins = 0;
       while (__hip_atomic_load(&bar[32], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT) == gen) {
         __builtin_amdgcn_s_sleep(1);
-        if (++spins > (1u << 27)) {
-          atomicOr(err, 1u);
+        if (wait_gives_up(bar, ++spins, lim, err)) {
+          aborted = 1;
           break;
         }
       }
@@ -116,9 +153,9 @@ __device__ __forceinline__ void grid_arrive_wait(u32 *bar, u32 n, u32 member, u3
 }
 
 __device__ __forceinline__ void grid_sync_coherent(u32 *bar, u32 n, u32 member, u32 &gen,
-                                                   u32 *err) {
+                                                   u32 *err, const u32 *limit, u32 &aborted) {
   wait_vmem();
-  grid_arrive_wait(bar, n, member, gen, err);
+  grid_arrive_wait(bar, n, member, gen, err, limit, aborted);
 }
 
 __device__ __forceinline__ u32 read_gen(const u32 *bar) {

@@ -35,6 +35,7 @@ namespace jqd {
 
 __device__ u32 g_barrier[kCohSlots * kCohSlotWords];
 __device__ u32 g_error;
+__device__ u32 g_wait_polls = kDefaultWaitPolls;  // bound of every wait here
 // progress words of the dataflow-synchronised launches: per barrier slot,
 // 256 words one 128-byte line apart (packed into 8 lines, every block's
 // polls and the producers' stores met on the same lines, and the 1M-gate
@@ -561,7 +562,6 @@ __device__ void run_partition(const ScanArgs &A, const u32 *__restrict__ bd,
 constexpr u32 kFlowMaxBlocks = 256;
 constexpr u32 kFlowMaxStages = 120;
 constexpr u32 kFlowNone = 0xFFFFFFFFu;
-constexpr u32 kFlowSpins = 1u << 22;
 constexpr u32 kFlowMinCycles = 8;
 
 struct FlowBuild {
@@ -652,14 +652,16 @@ __global__ __launch_bounds__(256) void flow_dep_kernel(FlowBuild F) {
 
 // Publishes `done` as block b's progress once every wave has waited for its
 // stores, then wave 0 polls the progress of the producers in `row` until
-// each reaches base + its offset.  The wait is bounded: after kFlowSpins
-// polls it raises `err` and the block stops waiting for the rest of the
-// launch, so the grid always drains.
+// each reaches base + its offset.  The wait is bounded like the grid
+// barrier's (the poll limit, and the launch's abort word in `bar` read every
+// 256 polls); an abort sets the caller's LDS word s_aborted to 1.
 __device__ __forceinline__ void flow_publish_wait(u32 *prog, u32 b, u32 done, u32 row,
-                                                  u32 base, bool &failed, u32 *err) {
+                                                  u32 base, u32 *bar, u32 *err,
+                                                  u32 &s_aborted) {
   __syncthreads();
   if (threadIdx.x == 0) st_coherent(&prog[b * kProgStride], done);
-  if (threadIdx.x < 64 && !failed && row != 0x80808080u) {
+  bool aborted = false;
+  if (threadIdx.x < 64 && row != 0x80808080u) {
     const u32 lane = threadIdx.x;
     u32 tg[4];
     bool need[4];
@@ -669,6 +671,7 @@ __device__ __forceinline__ void flow_publish_wait(u32 *prog, u32 b, u32 done, u3
       need[j] = d != 0x80u;
       tg[j] = base + (u32)(int)(signed char)d;
     }
+    const u32 lim = g_wait_polls;
     u32 spins = 0;
     for (;;) {
       u32 p[4];
@@ -685,13 +688,20 @@ __device__ __forceinline__ void flow_publish_wait(u32 *prog, u32 b, u32 done, u3
         }
       if (!__any(pend)) break;
       __builtin_amdgcn_s_sleep(1);
-      if (++spins > kFlowSpins) {
-        if (lane == 0) atomicOr(err, 1u);
-        failed = true;
+      // the outcome is wave-uniform (every lane counts the same polls and
+      // reads the same word); one lane raises the abort
+      if (++spins > lim) {
+        if (lane == 0) abort_launch(bar, err);
+        aborted = true;
+        break;
+      }
+      if ((spins & 255u) == 0 && ld_coherent(&bar[kAbortWord]) != 0) {
+        aborted = true;
         break;
       }
     }
   }
+  if (threadIdx.x == 0 && aborted) s_aborted = 1;
   asm volatile("" ::: "memory");
   __syncthreads();
 }
@@ -705,7 +715,9 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
   __shared__ u32 st_v[256], st_x[256], hin_v[256], hin_x[256];
   __shared__ u32 wo_v[256], wo_x[256];
   __shared__ u16 st_arr[256], wo_arr[256];
+  __shared__ u32 s_aborted;  // set by a wait when the launch has aborted
   const int t = threadIdx.x;
+  if (t == 0) s_aborted = 0;
   st_x[t] = 0;
   hin_x[t] = 0;
   wo_x[t] = 0;
@@ -722,15 +734,17 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
   // value-only launches whose producer table was built synchronise by
   // dataflow; the flag is the same word for every block
   const bool flow = !TIMING && !PROFILE && A.dep != nullptr && *A.dep_bad == 0;
-  bool flow_failed = false;
   if (flow) {
     if (t == 0) st_coherent(&A.prog[b * kProgStride], 0u);
-    grid_sync_coherent(A.bar, nb, b, gen, &g_error);
+    grid_sync_coherent(A.bar, nb, b, gen, &g_error, &g_wait_polls, s_aborted);
   }
   PartHdr h = M ? load_hdr(bd, A.blocks_start[b]) : no_hdr();
   PartPre P;
   if (h.num_stages) load_pre(P, bd, h, A.states);
   for (u32 c = 0; c < A.num_cycles; ++c) {
+    // an aborted launch (a bounded wait gave up; the waits that follow end
+    // within 256 polls each) leaves at the next cycle
+    if (s_aborted) return;
     const u32 *in = A.states + (size_t)c * A.state_size;
     u32 *out = A.states + (size_t)(c + 1) * A.state_size;
     for (u32 m = 0; m < M; ++m) {
@@ -766,10 +780,10 @@ __global__ __launch_bounds__(256) void scan_kernel(ScanArgs A) {
         h = hn;
       }
       if (flow)
-        flow_publish_wait(A.prog, b, c * M + m + 1, row, nc * M, flow_failed,
-                          &g_error);
+        flow_publish_wait(A.prog, b, c * M + m + 1, row, nc * M, A.bar, &g_error,
+                          s_aborted);
       else
-        grid_arrive_wait(A.bar, nb, b, gen, &g_error);
+        grid_arrive_wait(A.bar, nb, b, gen, &g_error, &g_wait_polls, s_aborted);
     }
   }
 }
@@ -791,7 +805,9 @@ __global__ __launch_bounds__(256) void cosim_kernel(ScanArgs A, cosim::CosimArgs
   __shared__ u32 st_v[256], st_x[256], hin_v[256], hin_x[256];
   __shared__ u32 wo_v[256], wo_x[256];
   __shared__ u16 st_arr[256], wo_arr[256];
+  __shared__ u32 s_aborted;  // set by a wait when the launch has aborted
   const int t = threadIdx.x;
+  if (t == 0) s_aborted = 0;
   st_x[t] = 0;
   hin_x[t] = 0;
   wo_x[t] = 0;
@@ -812,9 +828,11 @@ __global__ __launch_bounds__(256) void cosim_kernel(ScanArgs A, cosim::CosimArgs
     return x;
   };
   for (u32 e = 0; e <= C.num_edges; ++e) {
+    // an aborted launch (a bounded wait gave up) leaves at the next edge
+    if (s_aborted) return;
     cosim::between_phase(C, (u32)t, b, nb, e);
     if (e == C.num_edges) break;
-    grid_sync_coherent(A.bar, nb, b, gen, &g_error);
+    grid_sync_coherent(A.bar, nb, b, gen, &g_error, &g_wait_polls, s_aborted);
     PartHdr h = M ? hdr2(A.blocks_start[b]) : no_hdr();
     PartPre P;
     if (h.num_stages) load_pre(P, bd, h, in);
@@ -839,7 +857,7 @@ __global__ __launch_bounds__(256) void cosim_kernel(ScanArgs A, cosim::CosimArgs
         if (hn.num_stages) load_pre(P, bd, hn, in);
         h = hn;
       }
-      grid_arrive_wait(A.bar, nb, b, gen, &g_error);
+      grid_arrive_wait(A.bar, nb, b, gen, &g_error, &g_wait_polls, s_aborted);
     }
   }
 }
@@ -1057,6 +1075,18 @@ extern "C" int jq_max_resident_blocks(int kind, int *out_blocks) {
   return 0;
 }
 
+extern "C" usize jq_wait_limit(usize polls) {
+  u32 prev = 0;
+  if (hipMemcpyFromSymbol(&prev, HIP_SYMBOL(g_wait_polls), 4) != hipSuccess) return 0;
+  if (polls) {
+    const u32 v = polls > 0xFFFFFFFFull ? 0xFFFFFFFFu : (u32)polls;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_wait_polls), &v, 4) != hipSuccess ||
+        batched_set_wait_limit(v))
+      return 0;
+  }
+  return prev;
+}
+
 extern "C" int jq_scan_sync_mode(int mode) {
   if (mode < 0) return __atomic_load_n(&g_scan_sync, __ATOMIC_RELAXED);
   return __atomic_exchange_n(&g_scan_sync, mode ? 1 : 0, __ATOMIC_RELAXED);
@@ -1148,7 +1178,8 @@ extern "C" int jq_cosim_edges(usize num_blocks, usize num_major_stages,
     return fail("hipGetSymbolAddress(g_barrier) failed");
   A.bar = bar_base + kCohSlotWords * (__atomic_fetch_add(&g_slot, 1u, __ATOMIC_RELAXED) % kCohSlots);
   // the flash state's double buffer: words 64..87 of the launch's barrier
-  // slot, which the barrier does not use (generation at 32, counters from 256)
+  // slot, which the barrier does not use (generation at 32, abort word at
+  // 36, counters from 256)
   C.fsbuf = A.bar + 64;
   void *args[] = {&A, &C};
   const void *k = timing ? reinterpret_cast<const void *>(&cosim_kernel<true>)

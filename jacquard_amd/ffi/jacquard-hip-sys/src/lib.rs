@@ -226,6 +226,8 @@ extern "C" {
     /// No counterpart in the reference; see include/jacquard_hip.h.
     pub fn jq_scan_sync_mode(mode: c_int) -> c_int;
     /// No counterpart in the reference; see include/jacquard_hip.h.
+    pub fn jq_wait_limit(polls: usize) -> usize;
+    /// No counterpart in the reference; see include/jacquard_hip.h.
     pub fn jq_output_changes(
         states: *const u32,
         snapshot_offsets: *const usize,

@@ -235,3 +235,27 @@ def test_batched_lanes_from_input_vcds():
         rows = [i for i, (_, ts) in enumerate(offs) if ts != vcd.U64_MAX]
         got = vcd.write_lane_vcd(s, outs[k][rows], run.obs_lits, [offs[i][1] for i in rows])
         assert got == want, f"lane {k}"
+
+
+def test_wait_time_out_aborts_the_batched_launch():
+    """The batched kernel's time-out path, forced from the host with a
+    one-poll bound: the launch ends promptly with the device flag raised
+    (run raises and clears it), and with the bound restored a new run of
+    the same lanes matches the oracle."""
+    import time
+    from jacquard_amd import _native as N
+    L = N.hip()
+    nl, comp = design("c1", level_split=6, **CONFIG1)
+    s = comp.flatten(8)
+    seeds, probs, cycles = _lanes(40, 20, 80, 9)
+    prev = L.jq_wait_limit(1)
+    try:
+        t = time.time()
+        with pytest.raises(RuntimeError, match="timed out"):
+            BatchedRun(s, seeds, probs, cycles, groups=1).run()
+        print("aborted launch ended in", time.time() - t, "s", flush=True)
+        assert time.time() - t < 30
+    finally:
+        assert L.jq_wait_limit(prev) == 1
+    run = BatchedRun(s, seeds, probs, cycles, groups=1)
+    _check(s, run, seeds, probs, cycles, range(40))

@@ -176,6 +176,37 @@ def test_more_major_stages_than_the_dataflow_table_holds():
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("sync", [0, 1])
+def test_wait_time_out_aborts_the_launch_and_the_next_launch_is_exact(sync):
+    """The time-out path, forced from the host: with a one-poll bound nearly
+    every wait gives up, the first one marks the launch aborted, every
+    workgroup leaves at its next wait, and the launch ends promptly with the
+    device flag raised (run_scan raises and clears it).  With the bound
+    restored the next launch of the same script matches the oracle."""
+    import time
+    from jacquard_amd import _native as N
+    L = N.hip()
+    nl, comp = design("sync", level_split=4, seed=9, gates=40000, depth=48,
+                      dffs=3000, inputs=64, outputs=32, geometric_p=0.3, srams=3)
+    s = comp.flatten(48)
+    cycles = 200
+    st = stimulus.build_states(s, cycles, seed=18, p=0.3)
+    ref = st.copy()
+    oracle.run(s, ref, cycles)
+    with _sync_mode(sync):
+        prev = L.jq_wait_limit(1)
+        try:
+            t = time.time()
+            with pytest.raises(RuntimeError, match="timed out"):
+                sim.run_scan(s, st.copy(), cycles)
+            _say("aborted launch ended in", time.time() - t, "s")
+            assert time.time() - t < 30
+        finally:
+            assert L.jq_wait_limit(prev) == 1
+        got = sim.run_scan(s, st.copy(), cycles)["states"]
+    assert np.array_equal(got, ref)
+
+
 def test_timing_events():
     nl, comp = design("tm", level_split=6, seed=6, gates=8000, depth=30,
                       dffs=800, inputs=32, outputs=16, geometric_p=0.3)
