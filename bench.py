@@ -574,22 +574,22 @@ def bench_north(a, torch):
     1M-gate netlist (one graph replay) and the structured 16M-gate
     netlist's 64 lanes x 20K cycles with learned costs, unless --no-forms
     (--no-16m drops the last)."""
-    import ctypes
-    from jacquard_amd import _native as N
     from jacquard_amd.script import Compiled
     t = time.time()
     nl = netlist_4m(a)
-    comp = Compiled(nl, level_split=a.split)
-    log(f"north: netlist {nl.counts()} compiled in {time.time() - t:.1f}s")
-    res = ctypes.c_int(0)
-    N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
-    script = comp.flatten(min(max(comp.parts_per_stage()), res.value))
+    log(f"north: netlist {nl.counts()} generated in {time.time() - t:.1f}s")
+    # the plain drop-in at its measured best width (sim.tune_plain): the
+    # caps whose widest major stage fits the co-resident workgroups (at
+    # level split 6 on this netlist only the compiled default), then every
+    # partition held to one boomerang stage
+    from jacquard_amd import sim
+    _comp, script, plain_tune = sim.tune_plain(nl, a.split, caps=(2600, 1800), log=log)
     pa = argparse.Namespace(**vars(a))
     pa.steps, pa.warmup = a.plain_launches, 1
     p = bench_plain(pa, script, torch)
     plain = p["cycles"] / p["wall"]
     log(f"north: plain drop-in {plain:,.0f} cycles/s on the 4M-gate netlist")
-    comp_b = comp if a.batched_split == a.split else Compiled(nl, level_split=a.batched_split)
+    comp_b = Compiled(nl, level_split=a.batched_split)
     b = bench_batched(a, comp_b, torch, steps=a.steps, widths=(32, 48, 64))
     value = b["lane_cycles"] / b["wall"]
     log(f"north: batched {value:,.0f} lane-cycles/s, {value / plain:.1f}x the plain drop-in")
@@ -636,6 +636,7 @@ def bench_north(a, torch):
                         "script_bytes": int(bs.blocks_data.nbytes),
                         "verified_cycles": b["verified_cycles"]},
             "plain": {"lane_cycles_per_s": plain, "blocks": script.num_blocks,
+                      "partition_tune": plain_tune,
                       "cycles": p["cycles"], "major_stages": script.num_major_stages,
                       "partitions": script.num_parts,
                       "script_bytes": int(script.blocks_data.nbytes),
