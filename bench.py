@@ -610,6 +610,13 @@ def bench_north(a, torch):
                          f"{usable} usable by this process",
                "cpu_model": model, "nproc": online, "usable_cores": usable}
         log(f"cpu: {c['value']:.2f} lane-cycles/s over {c['cycles']} cycles")
+        # labelled apart from the one-thread port: the same restatement with
+        # each major stage's blocks spread over the host's cores, and the
+        # direct AIG cycle simulator on the same netlist
+        cpu["variants"] = v = cpu_variants(a, script)
+        log(f"cpu: OpenMP {v['port_openmp']['lane_cycles_per_s']:.2f} "
+            f"({v['port_openmp']['threads']} threads), direct AIG "
+            f"{v['direct_aig']['lane_cycles_per_s']:.2f} lane-cycles/s")
     bs = b["script"]
     line = {
         "metric": METRIC, "value": value, "unit": "lane-cycles/s", "n_gpus": 1,
