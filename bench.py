@@ -711,7 +711,8 @@ def forms_1m(a, torch):
 def bench_mult16m(a, torch):
     """The structured 16M-gate netlist: 64 replicated 16x16 AND-array
     multiplier tiles with pipeline DFFs plus a 2M-gate random control AIG,
-    depth 128, 1M DFFs, level split 8.  --lanes lanes (64) of --cycles cycles
+    depth 128, 1M DFFs, at --split levels per major stage (5 is this
+    netlist's fastest for the batched form).  --lanes lanes (64) of --cycles cycles
     (20K) run batched with the block assignment from the learned cost model
     (trained on >=50K samples of the kernel's own partition timings) and,
     for the makespan comparison, with the heuristic one; blocks per group
@@ -779,7 +780,8 @@ def bench_mult16m(a, torch):
 
 def forms_16m(a, torch):
     """The structured 16M-gate netlist reported beside the headline: 64 lanes
-    x 20,000 cycles through 2 groups x 128 blocks with the learned block
+    x 20,000 cycles through 2 groups x 128 blocks at level split 5 with the
+    learned block
     assignment (A/B against the heuristic; no merge and no tiling sweep,
     whose flattenings of a 0.9 GB script take tens of seconds each) in one
     graph replay, and the plain drop-in (compiled at level split 4) over
@@ -794,20 +796,24 @@ def forms_16m(a, torch):
     f.steps, f.warmup, f.cycles_per_step = 2, 1, 1000
     t = time.time()
     nl = Netlist.multiplier(seed=a.seed)
-    comp = Compiled(nl, level_split=8)
+    # 5 levels per major stage: 559 us per group cycle at 2 x 128 blocks
+    # against 593 at split 4, 655 at 3, 664 at 6, 740 at 2 and 844 at 8
+    # (24-cycle trials, heuristic assignment, one box)
+    comp = Compiled(nl, level_split=5)
     t_host = time.time() - t
     log(f"forms: 16M netlist {nl.counts()} generated and compiled in {t_host:.1f}s")
     f.batched_blocks = max(8, min(max(comp.parts_per_stage()),
                                   B.max_resident_batched() // f.batched_groups))
     rng = np.random.default_rng(a.seed + 200)
     sp = (rng.integers(1, 2**31, 64).astype(np.uint32), rng.uniform(0.02, 0.5, 64))
-    b = bench_batched(f, comp, torch, cycles=np.full(64, 20_000), split=8, steps=0,
+    b = bench_batched(f, comp, torch, cycles=np.full(64, 20_000), split=5, steps=0,
                       learned=True, merge=False, seeds_probs=sp)
     value = b["lane_cycles"] / b["wall"]
     res = ctypes.c_int(0)
     N.check(N.hip().jq_max_resident_blocks(0, ctypes.byref(res)))
     # the plain drop-in at its own best level split: 4 levels per major
-    # stage (32 stages) ran 2,870 cycles/s against 2,560 at split 8
+    # stage (32 stages) ran 2,870 cycles/s against 2,560 at split 8 and
+    # 910 at split 12
     comp_p = Compiled(nl, level_split=4)
     script = comp_p.flatten(min(max(comp_p.parts_per_stage()), res.value))
     p = bench_plain(f, script, torch)
@@ -816,7 +822,7 @@ def forms_16m(a, torch):
     return {"batched_16m": {
         "workload": f"structured netlist: {nl.counts()['num_and']} AND (64 16x16 "
                     f"multiplier tiles with pipeline DFFs + 2M-gate control AIG), "
-                    f"{nl.counts()['num_dffs']} DFF, level split 8; 64 lanes x 20000 "
+                    f"{nl.counts()['num_dffs']} DFF, level split 5; 64 lanes x 20000 "
                     f"cycles, toggle U[0.02, 0.5]; one hipGraph replay",
         "lane_cycles_per_s": value, "groups": b["groups"], "blocks_per_group": b["blocks"],
         "lane_cycles": b["lane_cycles"], "learned_cost_model": b["learned"],

@@ -366,7 +366,7 @@ def test_16m_gates_64_lanes_learned_costs():
     from jacquard_amd.script import Compiled
     from jacquard_amd.sim import device_error
     nl = Netlist.multiplier(seed=SEED)
-    comp = Compiled(nl, level_split=8)
+    comp = Compiled(nl, level_split=5)  # the bench's split for this netlist
     lanes, C = 64, 20_000
     G = 2
     W = max(8, min(max(comp.parts_per_stage()), B.max_resident_batched() // G))
