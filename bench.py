@@ -112,7 +112,7 @@ def parse():
     ap.add_argument("--no-width-tune", action="store_true",
                     help="batched form with the compiler's default partition width "
                          "instead of the measured choice (batched.tune_partition_width)")
-    ap.add_argument("--tune-splits", default="2,3,4",
+    ap.add_argument("--tune-splits", default="2,3,4,5",
                     help="level splits the batched form's tiling tune also tries "
                          "(comma-separated; empty: only --batched-split)")
     ap.add_argument("--no-learned", action="store_true",
