@@ -208,14 +208,15 @@ int jq_simulate_batched_stimulus(
  * scan kernel (kind 0) or the batched kernel (kind 1) on the current device. */
 int jq_max_resident_blocks(int kind, int *out_blocks);
 
-/* Device barrier time-out flag (no counterpart in the reference): *out is
- * nonzero when a grid barrier wait of a scan or batched launch gave up after
- * its bounded spin since the flag was last cleared.  Launches return before
- * their kernels finish, so callers read the flag after synchronising; the
- * results of a launch that timed out are not valid.  With clear != 0 the flag is
- * reset, and when it was set every barrier slot of the kind that timed out
- * is zeroed as well, so later launches start from a consistent barrier
- * state; clear only while no launch runs on the device. */
+/* Device wait time-out flag (no counterpart in the reference): *out is
+ * nonzero when a bounded wait (a grid barrier or a dataflow wait) of a scan,
+ * co-simulation or batched launch gave up (jq_wait_limit) since the flag
+ * was last cleared.  Launches return before their kernels finish, so
+ * callers read the flag after synchronising; the results of a launch that
+ * timed out are not valid.  With clear != 0 the flag is reset, and when it
+ * was set every barrier slot of the kind that timed out is zeroed as well
+ * (abort words included), so later launches start from a consistent
+ * barrier state; clear only while no launch runs on the device. */
 int jq_device_error_flag(u32 *out, int clear);
 
 /* Bound of the device waits (no counterpart in the reference, whose grid
@@ -225,10 +226,11 @@ int jq_device_error_flag(u32 *out, int clear);
  * flag of jq_device_error_flag and marks its launch aborted; the launch's
  * other waits notice within 256 polls and every workgroup leaves at its
  * next cycle, so a launch whose workgroups are not all resident ends soon
- * after one time-out.  polls > 0 sets the bound (capped at 2^32 - 1; set it
- * only while no launch runs); 0 only queries.  Returns the previous bound,
- * or 0 if the device could not be reached.  For hosts that want a shorter
- * bound and for tests of the time-out path. */
+ * after one time-out.  polls > 0 sets the bound of the current device's
+ * launches (capped at 2^32 - 1; each device keeps its own; set it only while
+ * no launch runs on that device); 0 only queries.  Returns the previous
+ * bound, or 0 if the device could not be reached.  For hosts that want a
+ * shorter bound and for tests of the time-out path. */
 usize jq_wait_limit(usize polls);
 
 /* Synchronisation of value-only scan launches (no counterpart in the
