@@ -41,12 +41,65 @@ struct Sram {
   uint32_t w_data_iv[32] = {};
 };
 
+// Structural-hash table of AND gates: key (lower operand literal << 32 |
+// higher) -> pin.  Open addressing with linear probing over a power-of-two
+// array, grown at half load: one probe sequence in two flat arrays instead
+// of a heap node per gate.  Keys are never 0 (both operands are literals
+// >= 2) and an AND's pin is never 0, so 0 marks empty slots and absence.
+class AndTable {
+ public:
+  uint32_t find(uint64_t key) const {
+    if (keys_.empty()) return 0;
+    for (size_t i = slot(key);; i = (i + 1) & (keys_.size() - 1)) {
+      if (keys_[i] == key) return vals_[i];
+      if (keys_[i] == 0) return 0;
+    }
+  }
+  void insert(uint64_t key, uint32_t pin) {
+    if (2 * (used_ + 1) > keys_.size()) grow();
+    size_t i = slot(key);
+    while (keys_[i] != 0 && keys_[i] != key) i = (i + 1) & (keys_.size() - 1);
+    used_ += keys_[i] == 0;
+    keys_[i] = key;
+    vals_[i] = pin;
+  }
+
+ private:
+  std::vector<uint64_t> keys_;
+  std::vector<uint32_t> vals_;
+  size_t used_ = 0;
+  // the splitmix64 output function (Steele, Lea and Flood, OOPSLA 2014)
+  static uint64_t mix(uint64_t k) {
+    k ^= k >> 30;
+    k *= 0xbf58476d1ce4e5b9ull;
+    k ^= k >> 27;
+    k *= 0x94d049bb133111ebull;
+    return k ^ (k >> 31);
+  }
+  size_t slot(uint64_t key) const { return (size_t)mix(key) & (keys_.size() - 1); }
+  void grow() {
+    std::vector<uint64_t> ok;
+    std::vector<uint32_t> ov;
+    ok.swap(keys_);
+    ov.swap(vals_);
+    keys_.assign(ok.empty() ? 1024 : 2 * ok.size(), 0);
+    vals_.assign(keys_.size(), 0);
+    for (size_t j = 0; j < ok.size(); ++j)
+      if (ok[j]) {
+        size_t i = slot(ok[j]);
+        while (keys_[i] != 0) i = (i + 1) & (keys_.size() - 1);
+        keys_[i] = ok[j];
+        vals_[i] = ov[j];
+      }
+  }
+};
+
 struct Aig {
   uint32_t num_pins = 0;
   std::vector<uint8_t> drv{kTie0};
   std::vector<uint32_t> a{0}, b{0};  // AND operands (literals)
   std::vector<uint32_t> ref{0};      // port / dff / sram index of a source
-  std::unordered_map<uint64_t, uint32_t> and_cache;
+  AndTable and_cache;
 
   std::vector<uint32_t> pos;  // primary outputs (literals), unique
   std::vector<std::string> po_names;
@@ -84,12 +137,11 @@ struct Aig {
     if (y == 1) return x;
     if (x > y) std::swap(x, y);
     uint64_t key = (uint64_t)x << 32 | y;
-    auto it = and_cache.find(key);
-    if (it != and_cache.end()) return it->second << 1;
+    if (uint32_t hit = and_cache.find(key)) return hit << 1;
     uint32_t p = add_pin(kAnd);
     a[p] = x;
     b[p] = y;
-    and_cache.emplace(key, p);
+    and_cache.insert(key, p);
     return p << 1;
   }
 

@@ -270,18 +270,20 @@ void *jqh_flatten(void *aig, void *compiled, uint32_t num_blocks, int xprop,
     delete h;
     return nullptr;
   }
-  for (auto &kv : h->s.input_map) {
-    h->in_pin.push_back(kv.first);
-    h->in_pos.push_back(kv.second);
-  }
-  for (auto &kv : h->s.output_map) {
-    h->out_lit.push_back(kv.first);
-    h->out_pos.push_back(kv.second);
-  }
-  for (auto &kv : h->s.staged_io_map) {
-    h->staged_pin.push_back(kv.first);
-    h->staged_pos.push_back(kv.second);
-  }
+  // the maps as (key, value) arrays in key order; the dense arrays they
+  // come from are not needed after flattening
+  auto pairs = [](std::vector<uint32_t> &dense, std::vector<uint32_t> &key,
+                  std::vector<uint32_t> &val) {
+    for (uint32_t k = 0; k < dense.size(); ++k)
+      if (dense[k] != kNone) {
+        key.push_back(k);
+        val.push_back(dense[k]);
+      }
+    std::vector<uint32_t>().swap(dense);
+  };
+  pairs(h->s.input_map, h->in_pin, h->in_pos);
+  pairs(h->s.output_map, h->out_lit, h->out_pos);
+  pairs(h->s.staged_io_map, h->staged_pin, h->staged_pos);
   for (auto x : h->s.part_x) h->part_x.push_back(x);
   for (auto &r : h->s.parts) {
     h->part_rec.push_back(r.major);
@@ -300,7 +302,8 @@ void jqh_script_free(void *h) { delete static_cast<ScriptH *>(h); }
 //        len(input_layout), len(input_map), len(output_map), len(staged),
 //        num_parts, timing, num_srams]
 void jqh_script_info(void *h, uint64_t *o) {
-  auto *s = &static_cast<ScriptH *>(h)->s;
+  auto *x = static_cast<ScriptH *>(h);
+  auto *s = &x->s;
   o[0] = s->num_blocks;
   o[1] = s->num_major_stages;
   o[2] = s->reg_io_state_size;
@@ -310,9 +313,9 @@ void jqh_script_info(void *h, uint64_t *o) {
   o[6] = s->xprop;
   o[7] = s->xprop_state_offset;
   o[8] = s->input_layout.size();
-  o[9] = s->input_map.size();
-  o[10] = s->output_map.size();
-  o[11] = s->staged_io_map.size();
+  o[9] = x->in_pin.size();
+  o[10] = x->out_lit.size();
+  o[11] = x->staged_pin.size();
   o[12] = s->parts.size();
   o[13] = s->timing;
   o[14] = s->sram_offsets.size();

@@ -700,9 +700,22 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
   out.stages.assign(nmajor, MajorStage());
   out.level_split = split;
   out.opt = opt;
+  // the AND nodes of each major stage in ascending pin order (fan-ins
+  // first), so that the per-stage passes below visit only their own nodes
+  std::vector<uint32_t> stage_first(nmajor + 1, 0), stage_nodes;
   for (uint32_t p = 1; p <= g.num_pins; ++p) {
     int s = stage_of(p);
+    if (s >= 0) ++stage_first[s + 1];
     if (s >= 0 && last_use[p] > s) out.stages[s].staged_out.push_back(p);
+  }
+  for (uint32_t m = 0; m < nmajor; ++m) stage_first[m + 1] += stage_first[m];
+  stage_nodes.resize(stage_first[nmajor]);
+  {
+    std::vector<uint32_t> fill(stage_first.begin(), stage_first.end() - 1);
+    for (uint32_t p = 1; p <= g.num_pins; ++p) {
+      int s = stage_of(p);
+      if (s >= 0) stage_nodes[fill[s]++] = p;
+    }
   }
   for (size_t e = 0; e < ne; ++e) out.stages[ep_stage[e]].orig_endpoints.push_back((uint32_t)e);
 
@@ -713,16 +726,57 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
   size_t total_parts = 0, total_bstages = 0, total_nodes = 0;
 
   // per-thread mapping contexts (stamp arrays over all pins), made on first use
-  std::vector<std::unique_ptr<MapCtx>> ctxs;
+  int max_threads = 1;
+#ifdef _OPENMP
+  max_threads = omp_get_max_threads();
+#endif
+  std::vector<std::unique_ptr<MapCtx>> ctxs(max_threads);
 
-  for (uint32_t m = 0; m < nmajor; ++m) {
+  // Grouping is greedy and runs stage by stage on one thread; every group
+  // it closes is mapped as a task by the other threads meanwhile, so the
+  // grouping of later stages overlaps the mapping of earlier ones.  Errors
+  // are kept per stage and group and the first in stage order is rethrown.
+  std::vector<std::vector<EndpointInfo>> infos_all(nmajor);
+  std::vector<std::vector<std::vector<uint32_t>>> groups_all(nmajor);
+  std::vector<std::vector<std::vector<Partition>>> built_all(nmajor);
+  std::vector<std::vector<std::string>> err_all(nmajor);
+  std::string group_err;
+  uint32_t group_err_stage = nmajor;
+  // map endpoints `eps` of stage m into res, in halves when they do not fit
+  std::function<void(uint32_t, MapCtx &, const std::vector<uint32_t> &,
+                     std::vector<Partition> &)>
+      build = [&](uint32_t m, MapCtx &cx, const std::vector<uint32_t> &eps,
+                  std::vector<Partition> &res) {
+    const std::function<bool(uint32_t)> avail = [&stage_of, m](uint32_t p) {
+      return stage_of(p) < (int)m;
+    };
+    Partition part;
+    if (map_group(g, opt, out.stages[m], infos_all[m], avail, eps, cx, part) &&
+        (!opt.max_boomerang_stages || eps.size() <= 1 ||
+         part.stages.size() <= opt.max_boomerang_stages)) {
+      res.push_back(std::move(part));
+      return;
+    }
+    if (eps.size() <= 1)
+      throw std::runtime_error("compile: a single endpoint does not fit a partition");
+    std::vector<uint32_t> a(eps.begin(), eps.begin() + eps.size() / 2);
+    std::vector<uint32_t> b(eps.begin() + eps.size() / 2, eps.end());
+    build(m, cx, a, res);
+    build(m, cx, b, res);
+  };
+  const auto t_all = clk::now();
+
+#pragma omp parallel
+#pragma omp single
+  for (uint32_t m = 0; m < nmajor; ++m) try {
     auto t0 = clk::now();
-    MajorStage &ms = out.stages[m];
+    const MajorStage &ms = out.stages[m];
     auto avail_pin = [&](uint32_t p) { return stage_of(p) < (int)m; };
     // stage-relative levels (cone depth above this stage's inputs)
     // endpoints: staged outputs then original endpoints
     size_t nep = ms.num_endpoints();
-    std::vector<EndpointInfo> infos(nep);
+    std::vector<EndpointInfo> &infos = infos_all[m];
+    infos.resize(nep);
     for (size_t e = 0; e < nep; ++e) endpoint_outputs(g, ms, e, infos[e]);
 
     // greedy grouping of endpoints; a group's cone is tracked with stamps
@@ -752,13 +806,13 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
       }
     };
     // stage-relative level of every node of this stage
-    for (uint32_t p = 1; p <= g.num_pins; ++p) {
-      if (g.drv[p] != kAnd || stage_of(p) != (int)m) continue;
+    for (uint32_t k = stage_first[m]; k < stage_first[m + 1]; ++k) {
+      const uint32_t p = stage_nodes[k];
       uint32_t la = avail_pin(g.a[p] >> 1) ? 0 : lvl_m[g.a[p] >> 1];
       uint32_t lb = avail_pin(g.b[p] >> 1) ? 0 : lvl_m[g.b[p] >> 1];
       lvl_m[p] = 1 + std::max(la, lb);
     }
-    std::vector<std::vector<uint32_t>> group_eps;
+    std::vector<std::vector<uint32_t>> &group_eps = groups_all[m];
     std::vector<uint32_t> ge;
     auto flush = [&]() {
       if (!ge.empty()) group_eps.push_back(ge);
@@ -808,65 +862,44 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
     }
     flush();
     t_group += secs(t0);
-    t0 = clk::now();
-
-    // map every group (in parallel, each thread with its own stamp arrays);
-    // split a group in halves when it does not fit
-    std::vector<std::vector<Partition>> built(group_eps.size());
-    const std::function<bool(uint32_t)> avail = avail_pin;
-    std::function<void(MapCtx &, std::vector<uint32_t>, std::vector<Partition> &)> build =
-        [&](MapCtx &cx, std::vector<uint32_t> eps, std::vector<Partition> &res) {
-      Partition part;
-      if (map_group(g, opt, ms, infos, avail, eps, cx, part) &&
-          (!opt.max_boomerang_stages || eps.size() <= 1 ||
-           part.stages.size() <= opt.max_boomerang_stages)) {
-        res.push_back(std::move(part));
-        return;
-      }
-      if (eps.size() <= 1)
-        throw std::runtime_error("compile: a single endpoint does not fit a partition");
-      std::vector<uint32_t> a(eps.begin(), eps.begin() + eps.size() / 2);
-      std::vector<uint32_t> b(eps.begin() + eps.size() / 2, eps.end());
-      build(cx, a, res);
-      build(cx, b, res);
-    };
-    std::string par_err;
-#pragma omp parallel
-    {
-      int tid = 0;
-#ifdef _OPENMP
-      tid = omp_get_thread_num();
-#endif
-#pragma omp single
+    built_all[m].resize(group_eps.size());
+    err_all[m].resize(group_eps.size());
+    for (size_t gi = 0; gi < group_eps.size(); ++gi) {
+#pragma omp task firstprivate(m, gi)
       {
-        int nt = 1;
+        int tid = 0;
 #ifdef _OPENMP
-        nt = omp_get_num_threads();
+        tid = omp_get_thread_num();
 #endif
-        if ((int)ctxs.size() < nt) ctxs.resize(nt);
-      }
-#pragma omp for schedule(dynamic, 1)
-      for (int64_t gi = 0; gi < (int64_t)group_eps.size(); ++gi) {
         try {
-          if (!ctxs[tid]) {
-            auto c = std::make_unique<MapCtx>(g.num_pins + 1);
-            ctxs[tid] = std::move(c);
-          }
-          build(*ctxs[tid], group_eps[gi], built[gi]);
+          if (!ctxs[tid]) ctxs[tid] = std::make_unique<MapCtx>(g.num_pins + 1);
+          build(m, *ctxs[tid], groups_all[m][gi], built_all[m][gi]);
         } catch (const std::exception &e) {
-#pragma omp critical(jq_compile_err)
-          if (par_err.empty()) par_err = e.what();
+          err_all[m][gi] = e.what();
         }
       }
     }
-    if (!par_err.empty()) throw std::runtime_error(par_err);
-    for (auto &v : built)
+  } catch (const std::exception &e) {
+    // grouping failed: no later stage is grouped (its tasks still drain)
+    group_err = e.what();
+    group_err_stage = m;
+    break;
+  }
+  // (the parallel region's closing barrier has waited for every task)
+  for (uint32_t m = 0; m < nmajor; ++m) {
+    if (m == group_err_stage) throw std::runtime_error(group_err);
+    for (const std::string &e : err_all[m])
+      if (!e.empty()) throw std::runtime_error(e);
+  }
+  t_map = secs(t_all) - t_group;
+  for (uint32_t m = 0; m < nmajor; ++m) {
+    MajorStage &ms = out.stages[m];
+    for (auto &v : built_all[m])
       for (auto &part : v) {
         total_bstages += part.stages.size();
         total_nodes += part.num_nodes;
         ms.parts.push_back(std::move(part));
       }
-    t_map += secs(t0);
     // sort by decreasing number of boomerang stages (src/pe.rs:1068)
     std::stable_sort(ms.parts.begin(), ms.parts.end(),
                      [](const Partition &x, const Partition &y) {
@@ -877,7 +910,8 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
        << ms.orig_endpoints.size() << " endpoints, " << ms.parts.size()
        << " partitions\n";
   }
-  lg << "compile time: grouping " << t_group << " s, mapping " << t_map << " s\n";
+  lg << "compile time: grouping " << t_group << " s, mapping " << t_map
+     << " s beyond the grouping it overlaps\n";
   lg << "total: " << total_parts << " partitions, " << total_bstages
      << " boomerang stages, " << total_nodes << " mapped AND evaluations ("
      << g.num_and() << " unique)\n";
@@ -1235,9 +1269,40 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
   const uint32_t nb = opt.num_blocks;
   s.num_blocks = nb;
   s.num_major_stages = (uint32_t)c.stages.size();
+  s.input_map.assign((size_t)g.num_pins + 1, kNone);
+  s.staged_io_map.assign((size_t)g.num_pins + 1, kNone);
+  s.output_map.assign(2 * ((size_t)g.num_pins + 1), kNone);
   for (uint32_t p = 1; p <= g.num_pins; ++p)
     if (g.drv[p] == kInput || g.drv[p] == kClockFlag) s.input_layout.push_back(p);
   for (uint32_t i = 0; i < s.input_layout.size(); ++i) s.input_map[s.input_layout[i]] = i;
+  // The per-partition loops below run in parallel; each records its
+  // errors, and the first partition's in stage order is rethrown, the one
+  // a serial pass would have met first.
+  auto rethrow_first = [](const std::vector<std::string> &err) {
+    for (const std::string &e : err)
+      if (!e.empty()) throw std::runtime_error(e);
+  };
+  // map writes of one partition's endpoints, applied in partition order
+  // once the stage's partitions are placed, so a key written by two
+  // partitions keeps the later one's value as in one serial pass
+  enum : uint8_t { kWStaged, kWOut, kWIn, kWDff, kWSram };
+  struct MapWrite {
+    uint8_t kind;
+    uint32_t key, val;
+  };
+  // the state bit a global read takes a pin from, from the previous
+  // snapshot (an input or register output) or written earlier this cycle
+  auto read_pos = [&s](uint32_t pin, bool &staged) -> uint32_t {
+    if (pin < s.input_map.size() && s.input_map[pin] != kNone) {
+      staged = false;
+      return s.input_map[pin];
+    }
+    if (pin < s.staged_io_map.size() && s.staged_io_map[pin] != kNone) {
+      staged = true;
+      return s.staged_io_map[pin];
+    }
+    throw std::runtime_error("flatten: a read pin is neither an input nor written out");
+  };
   const uint32_t const_zero_pos = (uint32_t)s.input_layout.size();
   uint32_t states_start = (uint32_t)((s.input_layout.size() + 31) / 32);
   if (const_zero_pos == states_start * 32) states_start += 1;
@@ -1276,8 +1341,11 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
     }
     auto &fps = fparts[m];
     fps.assign(ms.parts.size(), FlatPart());
-    // init_afters_writeouts (src/flatten.rs:470-586)
-    for (size_t i = 0; i < ms.parts.size(); ++i) {
+    std::vector<std::string> err(ms.parts.size());
+    // init_afters_writeouts (src/flatten.rs:470-586); each partition only
+    // writes its own FlatPart
+#pragma omp parallel for schedule(dynamic, 8)
+    for (size_t i = 0; i < ms.parts.size(); ++i) try {
       const Partition &pt = ms.parts[i];
       FlatPart &fp = fps[i];
       for (const BoomStage &bs : pt.stages)
@@ -1321,7 +1389,10 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
         throw std::runtime_error("flatten: partition exceeds 256 writeout words");
       for (uint32_t k = 0; k < fp.wo_pins.size(); ++k)
         if (fp.wo_pins[k] != kNone) fp.pin2pos[fp.wo_pins[k]] = k;
+    } catch (const std::exception &e) {
+      err[i] = e.what();
     }
+    rethrow_first(err);
     // state and SRAM allocation in block-affinity order
     // (src/flatten.rs:1247-1278)
     for (auto &blk : bp)
@@ -1332,10 +1403,14 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
         fp.sram_start = sum_sram;
         sum_sram += fp.num_srams * kSramWords;
       }
-    // make_inputs_outputs (src/flatten.rs:694-858)
-    for (size_t i = 0; i < ms.parts.size(); ++i) {
+    // make_inputs_outputs (src/flatten.rs:694-858): placements in each
+    // partition's own FlatPart in parallel, map writes recorded
+    std::vector<std::vector<MapWrite>> wr(ms.parts.size());
+#pragma omp parallel for schedule(dynamic, 8)
+    for (size_t i = 0; i < ms.parts.size(); ++i) try {
       const Partition &pt = ms.parts[i];
       FlatPart &fp = fps[i];
+      std::vector<MapWrite> &w = wr[i];
       fp.sd.init();
       fp.ck.init();
       fp.cnt_dup = 0;
@@ -1343,39 +1418,39 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
       for (uint32_t e : pt.endpoints) {
         if (e < ms.staged_out.size()) {
           uint32_t p = ms.staged_out[e];
-          s.staged_io_map[p] = fp.state_start * 32 + get_or_place(fp, p << 1, 1);
+          w.push_back({kWStaged, p, fp.state_start * 32 + get_or_place(fp, p << 1, 1)});
           continue;
         }
         size_t oe = ms.orig_endpoints[e - ms.staged_out.size()];
         if (oe < g.pos.size()) {
           uint32_t lit = g.pos[oe];
           if (lit <= 1) {
-            s.output_map[lit] = 0;
+            w.push_back({kWOut, lit, 0});
             continue;
           }
-          s.output_map[lit] = fp.state_start * 32 + get_or_place(fp, lit, 1);
+          w.push_back({kWOut, lit, fp.state_start * 32 + get_or_place(fp, lit, 1)});
         } else if (oe < g.pos.size() + g.dffs.size()) {
           size_t di = oe - g.pos.size();
           const Dff &d = g.dffs[di];
           if (d.d_iv <= 1) {
-            s.input_map[d.q] = const_zero_pos;
+            w.push_back({kWIn, d.q, const_zero_pos});
             continue;
           }
           uint32_t pos = fp.state_start * 32 + get_or_place(fp, d.d_iv, d.en_iv);
-          s.output_map[d.d_iv] = pos;
-          s.input_map[d.q] = pos;
-          s.dff_data_pos[di] = pos;
+          w.push_back({kWOut, d.d_iv, pos});
+          w.push_back({kWIn, d.q, pos});
+          w.push_back({kWDff, (uint32_t)di, pos});
         } else {
           size_t si = oe - g.pos.size() - g.dffs.size();
           const Sram &r = g.srams[si];
           uint32_t rd_local = fp.num_writeouts - fp.num_srams + cur_sram;
           uint32_t rd_global = fp.state_start + rd_local;
-          s.sram_offsets[si] = fp.sram_start + cur_sram * kSramWords;
+          w.push_back({kWSram, (uint32_t)si, fp.sram_start + cur_sram * kSramWords});
           for (uint32_t k = 0; k < 32; ++k) {
             uint32_t d = r.rd_pin[k];
             if (!d) continue;
-            s.input_map[d] = rd_global * 32 + k;
-            s.output_map[d << 1] = rd_global * 32 + k;
+            w.push_back({kWIn, d, rd_global * 32 + k});
+            w.push_back({kWOut, d << 1, rd_global * 32 + k});
             place_ck(fp, rd_local * 32 + k, r.r_en_iv, 0);
           }
           uint32_t st = cur_sram * 128, perm, inv, set0;
@@ -1396,7 +1471,20 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
       }
       if ((fp.cnt_dup + 31) / 32 != fp.num_dups)
         throw std::runtime_error("flatten: duplicate count mismatch");
+    } catch (const std::exception &e) {
+      err[i] = e.what();
     }
+    rethrow_first(err);
+    for (const std::vector<MapWrite> &w : wr)
+      for (const MapWrite &x : w) {
+        std::vector<uint32_t> &to = x.kind == kWStaged ? s.staged_io_map
+                                    : x.kind == kWOut  ? s.output_map
+                                    : x.kind == kWIn   ? s.input_map
+                                    : x.kind == kWDff  ? s.dff_data_pos
+                                                       : s.sram_offsets;
+        if (x.key >= to.size()) throw std::runtime_error("flatten: map key out of range");
+        to[x.key] = x.val;
+      }
   }
   s.reg_io_state_size = sum_state;
   s.sram_storage_size = sum_sram;
@@ -1432,9 +1520,8 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
               }
         for (uint32_t pin : pt.stages[0].hier[0]) {
           if (pin == kNone) continue;
-          auto it = s.input_map.find(pin);
-          uint32_t pos = it != s.input_map.end() ? it->second : s.staged_io_map.at(pin);
-          r.reads.push_back(pos >> 5);
+          bool staged;
+          r.reads.push_back(read_pos(pin, staged) >> 5);
         }
         for (uint32_t e : pt.endpoints)
           if (e >= c.stages[m].staged_out.size() &&
@@ -1468,19 +1555,25 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
     s.xprop_state_offset = sum_state;
   }
 
-  // build_script per partition (src/flatten.rs:863-1130) and assemble
+  // build_script per partition (src/flatten.rs:863-1130): the sections of
+  // all stages' partitions are independent (the maps are only read), so
+  // they are built in one parallel loop, then assembled in block order
+  std::vector<std::pair<uint32_t, uint32_t>> work;  // (major stage, partition)
+  std::vector<size_t> first(c.stages.size(), 0);    // work index of (m, 0)
   for (size_t m = 0; m < c.stages.size(); ++m) {
-    const MajorStage &ms = c.stages[m];
-    std::vector<std::vector<uint32_t>> scripts(ms.parts.size());
-    std::vector<std::vector<std::pair<uint64_t, std::vector<uint32_t>>>> patches(ms.parts.size());
-    // partitions are independent here (shared maps are only read), so their
-    // sections are built in parallel; the first error is rethrown after
-    std::string par_err;
+    first[m] = work.size();
+    for (size_t i = 0; i < c.stages[m].parts.size(); ++i)
+      work.push_back({(uint32_t)m, (uint32_t)i});
+  }
+  std::vector<std::vector<uint32_t>> scripts(work.size());
+  std::vector<std::vector<std::pair<uint64_t, std::vector<uint32_t>>>> patches(work.size());
+  std::vector<std::string> berr(work.size());
 #pragma omp parallel for schedule(dynamic, 1)
-    for (size_t i = 0; i < ms.parts.size(); ++i) try {
-      const Partition &pt = ms.parts[i];
+  for (size_t wi = 0; wi < work.size(); ++wi) try {
+      const size_t m = work[wi].first, i = work[wi].second;
+      const Partition &pt = c.stages[m].parts[i];
       const FlatPart &fp = fparts[m][i];
-      std::vector<uint32_t> &sc = scripts[i];
+      std::vector<uint32_t> &sc = scripts[wi];
       sc.reserve(256 + 5120 * (pt.stages.size() + 2) + 1024);
       sc.push_back((uint32_t)pt.stages.size());
       sc.push_back(0);
@@ -1511,13 +1604,9 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
       std::map<uint32_t, uint32_t> taken;  // word|flag -> mask
       for (uint32_t pin : pt.stages[0].hier[0]) {
         if (pin == kNone) continue;
-        auto it = s.input_map.find(pin);
-        if (it != s.input_map.end()) {
-          taken[it->second >> 5] |= 1u << (it->second & 31);
-        } else {
-          uint32_t pos = s.staged_io_map.at(pin);
-          taken[(pos >> 5) | 0x80000000u] |= 1u << (pos & 31);
-        }
+        bool staged;
+        const uint32_t pos = read_pos(pin, staged);
+        taken[(pos >> 5) | (staged ? 0x80000000u : 0u)] |= 1u << (pos & 31);
       }
       std::vector<std::pair<uint32_t, uint32_t>> ent(taken.begin(), taken.end());
       std::vector<std::vector<std::pair<uint32_t, uint32_t>>> th;
@@ -1545,17 +1634,12 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
             }
         }
       }
-      std::vector<uint32_t> prev_pos;  // pin -> position map of previous stage
       std::unordered_map<uint32_t, uint16_t> last_pos;
       for (uint32_t pin : pt.stages[0].hier[0]) {
         if (pin == kNone || last_pos.count(pin)) continue;
-        auto it = s.input_map.find(pin);
-        uint64_t key;
-        if (it != s.input_map.end())
-          key = (uint64_t)it->second;
-        else
-          key = 1ull << 40 | s.staged_io_map.at(pin);
-        last_pos[pin] = local.at(key);
+        bool staged;
+        const uint32_t pos = read_pos(pin, staged);
+        last_pos[pin] = local.at((uint64_t)staged << 40 | pos);
       }
       for (size_t j = 0; j < pt.stages.size(); ++j) {
         const BoomStage &bs = pt.stages[j];
@@ -1601,7 +1685,7 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
           sc.push_back(xa[t]);
           sc.push_back(xb[t]);
           sc.push_back(ob[t]);
-          if (!tpins[t].empty()) patches[i].push_back({sc.size(), tpins[t]});
+          if (!tpins[t].empty()) patches[wi].push_back({sc.size(), tpins[t]});
           sc.push_back(0);
         }
         last_pos.clear();
@@ -1630,36 +1714,49 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
       };
       emit(fp.sd, false);
       emit(fp.ck, true);
+      // the partition's tables are not read again: freed on this thread
+      { FlatPart done = std::move(fparts[m][i]); }
     } catch (const std::exception &e) {
-#pragma omp critical(jq_flatten_err)
-      if (par_err.empty()) par_err = e.what();
+      berr[wi] = e.what();
     }
-    if (!par_err.empty()) throw std::runtime_error(par_err);
+  rethrow_first(berr);
+  // every block's partitions in their order (a one-header dummy for an
+  // empty block), the block's last header flagged: offsets in order, then
+  // the copies in parallel
+  uint64_t total = 0;
+  for (size_t m = 0; m < c.stages.size(); ++m)
     for (uint32_t blk = 0; blk < nb; ++blk) {
-      s.blocks_start.push_back(s.blocks_data.size());
+      s.blocks_start.push_back(total);
       const auto &lst = blocks_parts[m][blk];
       if (lst.empty()) {
-        std::vector<uint32_t> dummy(kThreads, 0);
-        dummy[1] = 1;
-        s.blocks_data.insert(s.blocks_data.end(), dummy.begin(), dummy.end());
+        total += kThreads;
         continue;
       }
-      uint64_t last_st = 0;
-      for (size_t k = 0; k < lst.size(); ++k) {
-        uint32_t pid = lst[k];
-        uint64_t base = s.blocks_data.size();
-        if (k + 1 == lst.size()) last_st = base;
-        s.blocks_data.insert(s.blocks_data.end(), scripts[pid].begin(), scripts[pid].end());
-        for (auto &pp : patches[pid]) s.delay_patch.push_back({base + pp.first, pp.second});
-        s.parts.push_back({(uint32_t)m, blk, pid, base, (uint64_t)scripts[pid].size()});
+      for (uint32_t pid : lst) {
+        const size_t wi = first[m] + pid;
+        for (auto &pp : patches[wi])
+          s.delay_patch.push_back({total + pp.first, std::move(pp.second)});
+        s.parts.push_back({(uint32_t)m, blk, pid, total, (uint64_t)scripts[wi].size()});
         if (opt.xprop) s.part_x.push_back(px[m][pid]);
+        total += scripts[wi].size();
       }
-      s.blocks_data[last_st + 1] = 1;
     }
-    std::vector<std::vector<uint32_t>>().swap(scripts);
+  s.blocks_start.push_back(total);
+  s.blocks_data.assign(total + kThreads * 8, 0);
+  const size_t nblk = c.stages.size() * (size_t)nb;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (size_t q = 0; q < nblk; ++q) {
+    const auto &lst = blocks_parts[q / nb][q % nb];
+    uint32_t *dst = s.blocks_data.data() + s.blocks_start[q], *last = dst;
+    for (uint32_t pid : lst) {
+      std::vector<uint32_t> &sc = scripts[first[q / nb] + pid];
+      last = dst;
+      std::memcpy(dst, sc.data(), sc.size() * sizeof(uint32_t));
+      dst += sc.size();
+      std::vector<uint32_t>().swap(sc);
+    }
+    last[1] = 1;  // also the dummy header's "last" word
   }
-  s.blocks_start.push_back(s.blocks_data.size());
-  s.blocks_data.resize(s.blocks_data.size() + kThreads * 8, 0);
   (void)c;
 }
 

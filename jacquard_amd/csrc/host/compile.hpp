@@ -73,10 +73,12 @@ struct Script {
   uint32_t reg_io_state_size = 0;
   uint32_t sram_storage_size = 0;  // u32 entries
   std::vector<uint32_t> input_layout;
-  std::unordered_map<uint32_t, uint32_t> input_map;      // pin -> bit
-  std::unordered_map<uint32_t, uint32_t> output_map;     // literal -> bit
-  std::unordered_map<uint32_t, uint32_t> staged_io_map;  // pin -> bit
-  std::vector<uint32_t> sram_offsets;                    // per SRAM index
+  // the reference's three maps as dense arrays, kNone where a key has no
+  // entry: indexed by pin (input_map, staged_io_map) or literal (output_map)
+  std::vector<uint32_t> input_map;      // pin -> bit
+  std::vector<uint32_t> output_map;     // literal -> bit
+  std::vector<uint32_t> staged_io_map;  // pin -> bit
+  std::vector<uint32_t> sram_offsets;   // per SRAM index
   bool xprop = false;
   uint32_t xprop_state_offset = 0;
   std::vector<uint8_t> part_x;  // flat over stages/blocks/parts

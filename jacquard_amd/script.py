@@ -173,7 +173,10 @@ class FlattenedScriptV1:
         nin, nout, nst = int(info[9]), int(info[10]), int(info[11])
         self.input_map = dict(zip(arr(ptrs[0], nin).tolist(), arr(ptrs[1], nin).tolist()))
         self.output_map = dict(zip(arr(ptrs[2], nout).tolist(), arr(ptrs[3], nout).tolist()))
-        self.staged_io_map = dict(zip(arr(ptrs[4], nst).tolist(), arr(ptrs[5], nst).tolist()))
+        # the largest of the three maps (every word written for a later
+        # major stage), read by none of this package's paths: kept as two
+        # arrays and made a dict when first read
+        self._staged = (arr(ptrs[4], nst), arr(ptrs[5], nst))
         self.num_parts = int(info[12])
         self.partition_x_capable = (
             arr(L.jqh_script_part_x(h), self.num_parts).astype(bool)
@@ -199,6 +202,15 @@ class FlattenedScriptV1:
             self._h = None
 
     # ------------------------------------------------------------------
+    @property
+    def staged_io_map(self):
+        """pin -> state bit of every value written for a later major stage
+        of the same cycle (src/flatten.rs:694-858)."""
+        if not hasattr(self, "_staged_map"):
+            pins, pos = self._staged
+            self._staged_map = dict(zip(pins.tolist(), pos.tolist()))
+        return self._staged_map
+
     @property
     def num_srams(self):
         return self.sram_storage_size // SRAM_WORDS
