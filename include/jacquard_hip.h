@@ -243,7 +243,9 @@ usize jq_wait_limit(usize polls);
  * derived table is the one allocation these launches make: 4 bytes per
  * state word plus 256 bytes per (block, major stage), taken with
  * hipMallocAsync on the launch's stream and released with hipFreeAsync
- * after it (if the allocation fails, the launch uses grid barriers).  mode 0: grid barriers everywhere.  mode < 0
+ * after it (if the allocation fails, the launch uses grid barriers, and so
+ * does a launch issued while its stream is being captured into a graph).
+ * mode 0: grid barriers everywhere.  mode < 0
  * only queries.  Returns the previous mode.  Results are identical in both
  * modes; set it only while no scan launch is being issued. */
 int jq_scan_sync_mode(int mode);

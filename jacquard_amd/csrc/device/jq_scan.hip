@@ -909,7 +909,9 @@ int launch_scan(ScanArgs &A, hipStream_t stream) {
 // null when the launch runs with grid barriers: the mode is off, the
 // script is outside the table's limits, the launch is shorter than
 // kFlowMinCycles (the table's ~30 us build is then comparable to the
-// barriers it saves), or the allocation failed.  A
+// barriers it saves), the stream is being captured into a graph (no
+// allocation or builder kernels go into the caller's graph), or the
+// allocation failed.  A
 // script that the builders find unusable (a writeout word written twice or
 // out of range, a same-cycle read of a word not written in an earlier
 // stage) raises the flag that the kernel reads, and the launch then uses
@@ -922,6 +924,12 @@ void *build_flow_tables(ScanArgs &A, hipStream_t stream) {
       M > kFlowMaxStages || A.num_cycles < kFlowMinCycles || A.state_size == 0 ||
       A.state_size > (usize(1) << 28))
     return nullptr;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cap) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (cap != hipStreamCaptureStatusNone) return nullptr;
   const size_t own_words = ((size_t)A.state_size + 63) & ~(size_t)63;
   const size_t dep_words = (size_t)nb * M * 64;
   void *buf = nullptr;
