@@ -58,7 +58,12 @@ void simulate_v1_noninteractive_timed_hip(
  * capacity of the caller's choosing (event_buffer must hold
  * 16 + 32*event_capacity bytes).  Returns 0 or a negative error code; the
  * message is available from jq_last_error().  For hosts that must not exit
- * on error (tests, services). */
+ * on error (tests, services).  A call on a stream that is being captured
+ * records the launch into the caller's graph as an ordinary kernel node
+ * with grid barriers (it fails if the blocks cannot all be resident at
+ * once); each replay runs the same cycles again over the same buffers, so
+ * the caller restores what a replay must start from (snapshot 0, SRAM
+ * contents, the event buffer's header). */
 int jq_simulate_scan(usize num_blocks, usize num_major_stages,
                      const usize *blocks_start, const u32 *blocks_data,
                      u32 *sram_data, u32 *sram_xmask, usize num_cycles,

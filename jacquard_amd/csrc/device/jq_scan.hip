@@ -896,9 +896,27 @@ int launch_scan(ScanArgs &A, hipStream_t stream) {
     A.prog = pb + (size_t)256 * kProgStride * ((A.bar - bar_base) / kCohSlotWords);
   }
   void *args[] = {&A};
-  hipError_t e = hipLaunchCooperativeKernel(
-      reinterpret_cast<const void *>(&scan_kernel<TIMING, PROFILE>),
-      dim3(A.num_blocks), dim3(256), args, 0, stream);
+  const void *kern = reinterpret_cast<const void *>(&scan_kernel<TIMING, PROFILE>);
+  // Under stream capture (a hipGraph of the launch) the launch is an
+  // ordinary kernel node: co-residency of the workgroups is checked here
+  // from the occupancy query, and the bounded waits turn a shortfall at a
+  // replay (other work holding CUs) into the error flag instead of a hang.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cap) != hipSuccess)
+    return fail("hipStreamIsCapturing failed");
+  hipError_t e;
+  if (cap == hipStreamCaptureStatusActive) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0) != hipSuccess)
+      return fail("occupancy query failed");
+    if ((long long)per * cus < (long long)A.num_blocks)
+      return fail("scan launch: more blocks than workgroups that can be resident at once");
+    e = hipLaunchKernel(kern, dim3(A.num_blocks), dim3(256), args, 0, stream);
+  } else {
+    e = hipLaunchCooperativeKernel(kern, dim3(A.num_blocks), dim3(256), args, 0, stream);
+  }
   if (e != hipSuccess)
     return fail(std::string("scan launch failed: ") + hipGetErrorString(e));
   return 0;

@@ -207,6 +207,70 @@ def test_wait_time_out_aborts_the_launch_and_the_next_launch_is_exact(sync):
     assert np.array_equal(got, ref)
 
 
+def test_captured_launch_replays_match_the_oracle():
+    """A value-only launch through jq_simulate_scan captured into a hipGraph
+    on a side stream: under capture it is an ordinary kernel node with grid
+    barriers (no dataflow table is built into the graph).  Before each of
+    three replays the snapshot buffer gets its input back, with a junk
+    pattern in every word the launch writes (a replay that skipped a word
+    would show it), the SRAMs are cleared, and an eager launch of another
+    script runs; every replay reproduces the oracle's snapshots and SRAM
+    contents, and no wait times out."""
+    import ctypes
+    import torch
+    from jacquard_amd import _native as N
+    L = N.hip()
+    nl, comp = design("cap", level_split=4, seed=9, gates=40000, depth=48,
+                      dffs=3000, inputs=64, outputs=32, geometric_p=0.3, srams=3)
+    s = comp.flatten(48)
+    cycles = 60
+    ss = s.effective_state_size()
+    st = stimulus.build_states(s, cycles, seed=19, p=0.3)
+    ref = st.copy()
+    sram_ref = np.zeros(s.sram_storage_size, np.uint32)
+    oracle.run(s, ref, cycles, sram=sram_ref)
+    # the launch's input: snapshot 0 and the pre-filled primary inputs, and
+    # a junk pattern in every writeout word of snapshots 1.. (the words
+    # past the inputs' n // 32 + 1), which each cycle rewrites in full
+    ws = len(s.input_layout) // 32 + 1
+    inp = st.copy().reshape(cycles + 1, ss)
+    inp[1:, ws:] = 0xA5C3F00D
+    inp = inp.reshape(-1)
+    d_bs = sim.to_device(np.ascontiguousarray(s.blocks_start, np.uint64))
+    d_bd = sim.to_device(np.ascontiguousarray(s.blocks_data, np.uint32))
+    d_in = sim.to_device(inp)
+    d_st = d_in.clone()
+    d_sram = torch.zeros(s.sram_storage_size, dtype=torch.int32, device="cuda")
+    d_sx = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _, small = design("c1", level_split=0, **CONFIG1)
+    s2 = small.flatten(8)
+    st2 = stimulus.build_states(s2, 50, seed=3, p=0.5)
+    ref2 = st2.copy()
+    oracle.run(s2, ref2, 50)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            rc = L.jq_simulate_scan(
+                s.num_blocks, s.num_major_stages, d_bs.data_ptr(), d_bd.data_ptr(),
+                d_sram.data_ptr(), d_sx.data_ptr(), cycles, ss, d_st.data_ptr(),
+                None, None, 0, 0, ctypes.c_void_p(side.cuda_stream))
+    torch.cuda.current_stream().wait_stream(side)
+    N.check(rc)
+    for k in range(3):
+        d_st.copy_(d_in)
+        d_sram.zero_()
+        assert np.array_equal(sim.run_scan(s2, st2.copy(), 50)["states"], ref2)
+        _say("replay", k)
+        g.replay()
+        torch.cuda.synchronize()
+        assert not sim.device_error(), f"replay {k}: a wait timed out"
+        assert np.array_equal(sim.from_device(d_st, np.uint32), ref), f"replay {k}"
+        assert np.array_equal(sim.from_device(d_sram, np.uint32), sram_ref), f"replay {k}"
+
+
 def test_timing_events():
     nl, comp = design("tm", level_split=6, seed=6, gates=8000, depth=30,
                       dffs=800, inputs=32, outputs=16, geometric_p=0.3)
