@@ -200,3 +200,43 @@ def test_split_partitions_vs_direct(xprop):
                  dff_init_x=None if pick is None else pick.astype(np.uint8))
     with pytest.raises(RuntimeError):
         comp.split([(0, 0), (0, 0)])
+
+
+def _wide_and_verilog(n):
+    """A module whose one output is the AND of n inputs, as a balanced tree
+    of AND2 cells (depth ceil(log2 n))."""
+    lines = [f"module wide(x, y);", f"  input [{n - 1}:0] x;", "  output y;"]
+    level = [f"x[{i}]" for i in range(n)]
+    k = 0
+    while len(level) > 1:
+        nxt = []
+        for i in range(0, len(level) - 1, 2):
+            w = f"w{k}"
+            k += 1
+            lines.append(f"  wire {w};")
+            lines.append(f"  AND2_00_0 g{k} (.A({level[i]}), .B({level[i + 1]}), .Y({w}));")
+            nxt.append(w)
+        if len(level) % 2:
+            nxt.append(level[-1])
+        level = nxt
+    lines.append(f"  assign y = {level[0]};")
+    lines.append("endmodule")
+    return "\n".join(lines) + "\n"
+
+
+def test_an_endpoint_wider_than_a_partition_is_an_error_and_the_compiler_recovers():
+    """One output over 9,000 inputs in a single major stage needs more than
+    the 8,192 input bits of a partition: mapping fails in a worker task and
+    the compile raises (it must not end the process).  The same compiler
+    then compiles a netlist that fits, and its script is exact."""
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    wide = Netlist.from_verilog(text=_wide_and_verilog(9000))
+    with pytest.raises(RuntimeError, match="does not fit a partition"):
+        Compiled(wide, level_split=0)
+    fits = Netlist.from_verilog(text=_wide_and_verilog(4000))
+    s = Compiled(fits, level_split=0).flatten(1)
+    cycles = 6
+    st = stimulus.build_states(s, cycles, seed=2, p=0.999)
+    oracle.run(s, st, cycles)
+    direct_check(s, st, cycles, 2, 0.999)
