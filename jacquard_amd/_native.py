@@ -93,6 +93,12 @@ def host():
         "jqh_vcd_timestamps": (c_vp, [c_vp]),
         "jqh_vcd_free": (None, [c_vp]),
         "jqh_vcd_last_error": (c_cp, []),
+        "jqh_vcd_write_values": (c_vp, [c_vp, c_u64, c_u32, c_vp, c_cp, c_vp]),
+        "jqh_vcd_write_changes": (c_vp, [c_vp, c_vp, c_u64, c_u64, c_vp, c_u32, c_cp,
+                                         c_vp]),
+        "jqh_text_data": (c_vp, [c_vp]),
+        "jqh_text_size": (c_u64, [c_vp]),
+        "jqh_text_free": (None, [c_vp]),
         "jqh_constraint_buffer": (None, [c_vp, c_u32, ctypes.c_uint16,
                                          ctypes.c_uint16, c_vp]),
         "jqh_inject_delays": (c_u64, [c_vp, c_u64, c_vp, c_vp, c_vp, c_u64,

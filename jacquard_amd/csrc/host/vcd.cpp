@@ -12,6 +12,7 @@
 //   - 'x' / 'z' read as 0; real values and $comment blocks are skipped.
 #include <cstdint>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <string_view>
 #include <unordered_map>
@@ -177,6 +178,103 @@ void *jqh_vcd_parse_body(const char *body, uint64_t len, const char *codes,
   out->states.insert(out->states.end(), state.begin(), state.end());
   return out;
 }
+
+// ---- writers: the value-change section of an output VCD
+// (src/sim/vcd_io.rs:758-820).  Every row gets a "#<time>" line; a column
+// is written ("0", "1" or "x" then its identifier code) only when its value
+// differs from the one it last wrote (none before the first row).  Code i
+// is codes[code_off[i] .. code_off[i+1]).  The text goes into a handle
+// (jqh_text_data / _size / _free) or, for NULL, the error message.
+
+namespace {
+
+struct VcdText {
+  std::string s;
+};
+
+inline void put_u64(std::string &o, uint64_t v) {
+  char b[24];
+  int n = 0;
+  do {
+    b[n++] = (char)('0' + v % 10);
+    v /= 10;
+  } while (v);
+  while (n) o.push_back(b[--n]);
+}
+
+inline void put_change(std::string &o, uint32_t v, const char *codes,
+                       const uint64_t *code_off, uint32_t i) {
+  o.push_back(v == 3 ? 'x' : (char)('0' + v));
+  o.append(codes + code_off[i], code_off[i + 1] - code_off[i]);
+  o.push_back('\n');
+}
+
+}  // namespace
+
+// vals[r * n_cols + i] in {0, 1, 3 (X)}: column i's value at row r, emitted
+// at timestamps[r].
+void *jqh_vcd_write_values(const uint8_t *vals, uint64_t n_rows, uint32_t n_cols,
+                           const uint64_t *timestamps, const char *codes,
+                           const uint64_t *code_off) {
+  t_err.clear();
+  auto *out = new VcdText();
+  try {
+    std::vector<uint8_t> last(n_cols, 2);
+    out->s.reserve(n_rows * 8 + n_rows * (size_t)n_cols);
+    for (uint64_t r = 0; r < n_rows; ++r) {
+      out->s.push_back('#');
+      put_u64(out->s, timestamps[r]);
+      out->s.push_back('\n');
+      const uint8_t *row = vals + r * n_cols;
+      for (uint32_t i = 0; i < n_cols; ++i) {
+        const uint8_t v = row[i];
+        if (v != 0 && v != 1 && v != 3) throw std::runtime_error("vcd: value not 0, 1 or 3");
+        if (v == last[i]) continue;
+        last[i] = v;
+        put_change(out->s, v, codes, code_off, i);
+      }
+    }
+  } catch (const std::exception &e) {
+    t_err = e.what();
+    delete out;
+    return nullptr;
+  }
+  return out;
+}
+
+// Change records (output << 2 | value): row r owns counts[r] of them, in
+// order, emitted at timestamps[r] (the device change extraction's output).
+void *jqh_vcd_write_changes(const uint32_t *counts, const uint32_t *records,
+                            uint64_t n_records, uint64_t n_rows,
+                            const uint64_t *timestamps, uint32_t n_cols,
+                            const char *codes, const uint64_t *code_off) {
+  t_err.clear();
+  auto *out = new VcdText();
+  try {
+    out->s.reserve(n_rows * 8 + n_records * 4);
+    uint64_t k = 0;
+    for (uint64_t r = 0; r < n_rows; ++r) {
+      out->s.push_back('#');
+      put_u64(out->s, timestamps[r]);
+      out->s.push_back('\n');
+      if (counts[r] > n_records - k) throw std::runtime_error("vcd: counts exceed the records");
+      for (uint64_t e = k + counts[r]; k < e; ++k) {
+        const uint32_t rec = records[k], i = rec >> 2, v = rec & 3;
+        if (i >= n_cols || v == 2) throw std::runtime_error("vcd: bad change record");
+        put_change(out->s, v, codes, code_off, i);
+      }
+    }
+  } catch (const std::exception &e) {
+    t_err = e.what();
+    delete out;
+    return nullptr;
+  }
+  return out;
+}
+
+const char *jqh_text_data(void *h) { return static_cast<VcdText *>(h)->s.data(); }
+uint64_t jqh_text_size(void *h) { return static_cast<VcdText *>(h)->s.size(); }
+void jqh_text_free(void *h) { delete static_cast<VcdText *>(h); }
 
 uint64_t jqh_vcd_num_words(void *h) { return static_cast<VcdBody *>(h)->states.size(); }
 const uint32_t *jqh_vcd_states(void *h) { return static_cast<VcdBody *>(h)->states.data(); }

@@ -13,6 +13,7 @@ follow src/sim/vcd_io.rs:561-946 and the vcd-ng/rust-vcd text layout
 (identifier codes '!'..'~' in base 94; beyond 94 wires this encoding is the
 rust-vcd convention and is not pinned by any reference fixture).
 """
+import ctypes
 import re
 
 import numpy as np
@@ -287,44 +288,74 @@ def _header_text(timescale, scope, mapping):
     return lines
 
 
-def _values(states, rio, offset, mapping, xmask):
-    vals = []
-    for lit, pos, _ in mapping:
-        if pos is None:
-            vals.append(lit)
-            continue
-        w = states[offset + (pos >> 5)]
-        v = int(w >> np.uint32(pos & 31)) & 1
-        if xmask is not None and (int(xmask[offset + (pos >> 5)] >> np.uint32(pos & 31)) & 1):
-            vals.append(3)
-        else:
-            vals.append(v)
-    return vals
-
-
 _SYM = {0: "0", 1: "1", 3: "x"}
+
+
+def _codes(n):
+    """Identifier codes of columns 0..n-1, concatenated, and their offsets."""
+    codes = [id_code(i) for i in range(n)]
+    off = np.zeros(n + 1, np.uint64)
+    if n:
+        off[1:] = np.cumsum([len(c) for c in codes])
+    return "".join(codes).encode(), off
+
+
+def _native_text(h):
+    L = N.host()
+    if not h:
+        raise RuntimeError("vcd writer: " + L.jqh_vcd_last_error().decode())
+    try:
+        return ctypes.string_at(L.jqh_text_data(h), L.jqh_text_size(h)).decode()
+    finally:
+        L.jqh_text_free(h)
+
+
+def _text_from_values(header, vals, timestamps):
+    """Header lines, then the value-change section written natively from
+    vals[row, column] in {0, 1, 3}: '#t' per row, a column when it changes."""
+    vals = np.ascontiguousarray(vals, np.uint8)
+    ts = np.ascontiguousarray(np.asarray(timestamps, dtype=np.uint64).reshape(-1))
+    if vals.ndim != 2 or vals.shape[0] != ts.size:
+        raise ValueError("one row of values per timestamp")
+    codes, off = _codes(vals.shape[1])
+    h = N.host().jqh_vcd_write_values(vals.ctypes.data, vals.shape[0], vals.shape[1],
+                                      ts.ctypes.data, codes, off.ctypes.data)
+    return "\n".join(header) + "\n" + _native_text(h)
 
 
 def write_output_vcd(script, offsets_timestamps, states, timescale=(1, "ps"),
                      scope="gem_top_module", xmask_states=None):
     """Text of the output VCD (write_output_vcd[_xprop],
     src/sim/vcd_io.rs:729-820).  states / xmask_states: value-only buffers
-    laid out like parse_input_vcd's input_states."""
+    laid out like parse_input_vcd's input_states.  Output values are
+    gathered per column over all emitted snapshots; the value-change
+    section is written by the host library."""
     mapping = output_mapping(script)
-    lines = _header_text(timescale, scope, mapping)
-    last = [2] * len(mapping)
-    rio = script.reg_io_state_size
-    for offset, ts in offsets_timestamps:
-        if ts == U64_MAX:
+    rows = [(int(o), int(t)) for o, t in offsets_timestamps if t != U64_MAX]
+    vals = _output_values(mapping, [o for o, _ in rows], states, xmask_states)
+    return _text_from_values(_header_text(timescale, scope, mapping), vals,
+                             [t for _, t in rows])
+
+
+def _output_values(mapping, offsets, states, xmask_states):
+    """vals[row, column] in {0, 1, 3 (X)}: each output's value in the
+    snapshot at offsets[row] (value-only layout), gathered column by
+    column."""
+    offs = np.asarray(offsets, np.int64)
+    st = np.asarray(states, np.uint32)
+    xm = None if xmask_states is None else np.asarray(xmask_states, np.uint32)
+    vals = np.zeros((offs.size, len(mapping)), np.uint8)
+    for i, (lit, pos, _) in enumerate(mapping):
+        if pos is None:
+            vals[:, i] = lit
             continue
-        lines.append(f"#{ts}")
-        vals = _values(states, rio, offset, mapping, xmask_states)
-        for i, v in enumerate(vals):
-            if v == last[i]:
-                continue
-            last[i] = v
-            lines.append(f"{_SYM[v]}{id_code(i)}")
-    return "\n".join(lines) + "\n"
+        idx = offs + (pos >> 5)
+        sh = np.uint32(pos & 31)
+        col = ((st[idx] >> sh) & np.uint32(1)).astype(np.uint8)
+        if xm is not None:
+            col[((xm[idx] >> sh) & np.uint32(1)) != 0] = 3
+        vals[:, i] = col
+    return vals
 
 
 def write_lane_vcd(script, obs, obs_lits, timestamps, timescale=(1, "ps"),
@@ -337,17 +368,14 @@ def write_lane_vcd(script, obs, obs_lits, timestamps, timescale=(1, "ps"),
     not a constant or an input pass-through must be in obs_lits."""
     mapping = output_mapping(script)
     col = {lit: k for k, lit in enumerate(obs_lits)}
-    lines = _header_text(timescale, scope, mapping)
-    last = [2] * len(mapping)
-    for c, ts in enumerate(timestamps):
-        lines.append(f"#{ts}")
-        for i, (lit, pos, _) in enumerate(mapping):
-            v = lit if pos is None else int(obs[c, col[lit]])
-            if v == last[i]:
-                continue
-            last[i] = v
-            lines.append(f"{_SYM[v]}{id_code(i)}")
-    return "\n".join(lines) + "\n"
+    n = len(timestamps)
+    obs = np.asarray(obs)
+    if obs.shape[0] < n:
+        raise ValueError("fewer rows of outputs than timestamps")
+    vals = np.zeros((n, len(mapping)), np.uint8)
+    for i, (lit, pos, _) in enumerate(mapping):
+        vals[:, i] = lit if pos is None else obs[:n, col[lit]]
+    return _text_from_values(_header_text(timescale, scope, mapping), vals, timestamps)
 
 
 CONST0_BIT = 0xFFFFFFFF
@@ -402,19 +430,16 @@ def write_output_vcd_changes(script, offsets_timestamps, counts, records,
     (output_changes_device); the same text write_output_vcd produces from
     the full snapshots."""
     mapping = output_mapping(script)
-    lines = _header_text(timescale, scope, mapping)
-    codes = [id_code(i) for i in range(len(mapping))]
-    k = 0
-    r = 0
-    for _, ts in offsets_timestamps:
-        if ts == U64_MAX:
-            continue
-        lines.append(f"#{ts}")
-        for rec in records[r:r + int(counts[k])].tolist():
-            lines.append(f"{_SYM[rec & 3]}{codes[rec >> 2]}")
-        r += int(counts[k])
-        k += 1
-    return "\n".join(lines) + "\n"
+    ts = np.asarray([int(t) for _, t in offsets_timestamps if t != U64_MAX], np.uint64)
+    counts = np.ascontiguousarray(counts, np.uint32)
+    records = np.ascontiguousarray(records, np.uint32)
+    if counts.size < ts.size:
+        raise ValueError("one change count per emitted snapshot")
+    codes, off = _codes(len(mapping))
+    h = N.host().jqh_vcd_write_changes(counts.ctypes.data, records.ctypes.data, records.size,
+                                       ts.size, ts.ctypes.data, len(mapping), codes,
+                                       off.ctypes.data)
+    return "\n".join(_header_text(timescale, scope, mapping)) + "\n" + _native_text(h)
 
 
 def write_output_vcd_timed(script, offsets_timestamps, states, arrival_states,
@@ -431,11 +456,10 @@ def write_output_vcd_timed(script, offsets_timestamps, states, arrival_states,
         ps_per_unit = 1.0
     last = [2] * len(mapping)
     trans = []
-    for offset, ts in offsets_timestamps:
-        if ts == U64_MAX:
-            continue
-        vals = _values(states, script.reg_io_state_size, offset, mapping, xmask_states)
-        for i, v in enumerate(vals):
+    rows = [(int(o), int(t)) for o, t in offsets_timestamps if t != U64_MAX]
+    allv = _output_values(mapping, [o for o, _ in rows], states, xmask_states)
+    for r, (offset, ts) in enumerate(rows):
+        for i, v in enumerate(allv[r].tolist()):
             if v == last[i]:
                 continue
             last[i] = v

@@ -1,8 +1,9 @@
-"""Pure-Python restatement of the input-VCD value-change loop.
+"""Pure-Python restatement of the input-VCD value-change loop, and of the
+output VCD's value-change section.
 
-TEST INFRASTRUCTURE ONLY: the checker for the native streaming reader
-(jacquard_amd/csrc/host/vcd.cpp), imported by tests, never by the product
-package.
+TEST INFRASTRUCTURE ONLY: the checker for the native streaming reader and
+body writers (jacquard_amd/csrc/host/vcd.cpp), imported by tests, never by
+the product package.
 
 Follows parse_input_vcd's body loop (src/sim/vcd_io.rs:347-455): a snapshot
 is pushed at the first new timestamp after one at which a clock edge
@@ -93,3 +94,53 @@ def parse_body(body, code_map, flag_bits, rio, max_clock_edges=None):
             pending.append(bit)
     snaps.append(list(state))
     return np.array(snaps, np.uint64).astype(np.uint32).reshape(-1), offs
+
+
+# ---------------------------------------------------------------- writers
+# The value-change section of an output VCD (src/sim/vcd_io.rs:758-820),
+# restated as plain loops: the checker for the native body writers
+# (jqh_vcd_write_values / jqh_vcd_write_changes in csrc/host/vcd.cpp).
+
+def id_code(i):
+    """rust-vcd IdCode display: base-94 over '!'..'~', least significant
+    digit first, each further digit offset by one."""
+    s = []
+    while True:
+        s.append(chr(33 + i % 94))
+        if i < 94:
+            return "".join(s)
+        i = i // 94 - 1
+
+
+_SYM = {0: "0", 1: "1", 3: "x"}
+
+
+def body_from_values(rows, timestamps):
+    """rows[r][i] in {0, 1, 3}: every row gets '#t'; column i is written
+    when it differs from the value it last wrote (nothing before row 0)."""
+    out = []
+    last = None
+    for row, ts in zip(rows, timestamps):
+        if last is None:
+            last = [2] * len(row)
+        out.append(f"#{ts}")
+        for i, v in enumerate(row):
+            v = int(v)
+            if v == last[i]:
+                continue
+            last[i] = v
+            out.append(f"{_SYM[v]}{id_code(i)}")
+    return "".join(line + "\n" for line in out)
+
+
+def body_from_changes(counts, records, timestamps):
+    """Row r writes its counts[r] records (output << 2 | value) in order."""
+    out = []
+    k = 0
+    for r, ts in enumerate(timestamps):
+        out.append(f"#{ts}")
+        for rec in records[k:k + int(counts[r])]:
+            rec = int(rec)
+            out.append(f"{_SYM[rec & 3]}{id_code(rec >> 2)}")
+        k += int(counts[r])
+    return "".join(line + "\n" for line in out)

@@ -3,8 +3,8 @@ AddressSanitizer and UndefinedBehaviorSanitizer (gcc, host code only, no
 GPU code involved), running the CPU tests that exercise them: netlist
 generation, partitioning and multi-level mapping, partition merge, flatten,
 the validator, features, gather tables, the lane layout, timing words,
-the input VCD reader, and the oracle's interpreter, AIG simulator and
-planted-mistake builds.  Any invalid access or undefined behaviour aborts
+the input VCD reader, the output VCD writers, and the oracle's
+interpreter, AIG simulator and planted-mistake builds.  Any invalid access or undefined behaviour aborts
 the run."""
 import os
 import subprocess
@@ -17,7 +17,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HOST_SRC = ["aig.cpp", "netgen.cpp", "compile.cpp", "verilog.cpp", "capi.cpp", "vcd.cpp"]
 SAN = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"]
 TESTS = ["test_compile_direct.py", "test_batched_encoding.py", "test_timing_pins.py",
-         "test_vcd_reader.py", "test_oracle_kats.py", "test_word_symbols.py"]
+         "test_vcd_reader.py", "test_vcd_writer.py", "test_golden_vcd.py",
+         "test_capi_vcd_events.py", "test_oracle_kats.py", "test_word_symbols.py"]
 
 
 def _runtime(name):
