@@ -71,3 +71,27 @@ def test_writers_reject_malformed_input():
                                        1, 1, ts.ctypes.data, 4, codes, off.ctypes.data)
     with pytest.raises(RuntimeError, match="bad change record"):
         vcd._native_text(h)
+
+
+def test_lane_writer_matches_the_snapshot_writer():
+    """The batched form's writer (one row of observed outputs per cycle)
+    gives the text the plain path's writer gives from full snapshots, with
+    outputs read from an oracle run of the same stimulus (the GPU tests
+    compare the same two writers on device results)."""
+    import oracle
+    from helpers import CONFIG1, design
+    from jacquard_amd import stimulus
+    _, comp = design("c1", level_split=6, **CONFIG1)
+    s = comp.flatten(8)
+    c = 120
+    st = stimulus.build_states(s, c, seed=5, p=0.4)
+    oracle.run(s, st, c)
+    rio = s.reg_io_state_size
+    lits = [lit for lit, pos, _ in vcd.output_mapping(s) if pos is not None]
+    assert lits, "the design has outputs computed by the script"
+    snaps = st.reshape(c + 1, rio)[1:]
+    obs = np.stack([(snaps[:, s.output_map[l] >> 5] >> np.uint32(s.output_map[l] & 31)) & 1
+                    for l in lits], axis=1).astype(np.uint8)
+    ts = [10 * (k + 1) for k in range(c)]
+    want = vcd.write_output_vcd(s, [((k + 1) * rio, ts[k]) for k in range(c)], st)
+    assert vcd.write_lane_vcd(s, obs, lits, ts) == want
