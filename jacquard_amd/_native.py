@@ -96,6 +96,8 @@ def host():
         "jqh_vcd_write_values": (c_vp, [c_vp, c_u64, c_u32, c_vp, c_cp, c_vp]),
         "jqh_vcd_write_changes": (c_vp, [c_vp, c_vp, c_u64, c_u64, c_vp, c_u32, c_cp,
                                          c_vp]),
+        "jqh_vcd_write_timed": (c_vp, [c_vp, c_vp, c_u64, c_u32, c_vp, ctypes.c_double,
+                                       c_cp, c_vp]),
         "jqh_text_data": (c_vp, [c_vp]),
         "jqh_text_size": (c_u64, [c_vp]),
         "jqh_text_free": (None, [c_vp]),

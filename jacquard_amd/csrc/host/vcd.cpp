@@ -10,6 +10,8 @@
 //   - data pins change immediately; a clock pin's change sets its posedge
 //     flag (on a 0 -> 1 change) and is applied at the next new timestamp;
 //   - 'x' / 'z' read as 0; real values and $comment blocks are skipped.
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -263,6 +265,68 @@ void *jqh_vcd_write_changes(const uint32_t *counts, const uint32_t *records,
         if (i >= n_cols || v == 2) throw std::runtime_error("vcd: bad change record");
         put_change(out->s, v, codes, code_off, i);
       }
+    }
+  } catch (const std::exception &e) {
+    t_err = e.what();
+    delete out;
+    return nullptr;
+  }
+  return out;
+}
+
+// Timed form (src/sim/vcd_io.rs:831-946): a change of column i at row r
+// (against the value it last changed to; none before row 0) happens at
+// timestamps[r] + round(arrival_ps[r * n_cols + i] / ps_per_unit), halves
+// away from zero (0 when ps_per_unit <= 0); changes are stably sorted by
+// that time and a "#<time>" line opens each distinct time.  Times are kept
+// in 128 bits, so a shift past 2^64 - 1 prints as the sum.
+void *jqh_vcd_write_timed(const uint8_t *vals, const uint16_t *arrival_ps, uint64_t n_rows,
+                          uint32_t n_cols, const uint64_t *timestamps, double ps_per_unit,
+                          const char *codes, const uint64_t *code_off) {
+  t_err.clear();
+  auto *out = new VcdText();
+  try {
+    struct Change {
+      unsigned __int128 t;
+      uint32_t col;
+      uint8_t v;
+    };
+    std::vector<Change> ch;
+    std::vector<uint8_t> last(n_cols, 2);
+    for (uint64_t r = 0; r < n_rows; ++r)
+      for (uint32_t i = 0; i < n_cols; ++i) {
+        const uint8_t v = vals[r * n_cols + i];
+        if (v != 0 && v != 1 && v != 3) throw std::runtime_error("vcd: value not 0, 1 or 3");
+        if (v == last[i]) continue;
+        last[i] = v;
+        uint64_t units = 0;
+        if (ps_per_unit > 0) {
+          const double x = (double)arrival_ps[r * n_cols + i] / ps_per_unit;
+          units = (uint64_t)std::floor(x + 0.5);  // x >= 0
+        }
+        ch.push_back({(unsigned __int128)timestamps[r] + units, i, v});
+      }
+    std::stable_sort(ch.begin(), ch.end(),
+                     [](const Change &a, const Change &b) { return a.t < b.t; });
+    out->s.reserve(ch.size() * 6);
+    bool open = false;
+    unsigned __int128 cur = 0;
+    for (const Change &c : ch) {
+      if (!open || c.t != cur) {
+        out->s.push_back('#');
+        char b[48];
+        int n = 0;
+        unsigned __int128 t = c.t;
+        do {
+          b[n++] = (char)('0' + (int)(t % 10));
+          t /= 10;
+        } while (t);
+        while (n) out->s.push_back(b[--n]);
+        out->s.push_back('\n');
+        cur = c.t;
+        open = true;
+      }
+      put_change(out->s, c.v, codes, code_off, c.col);
     }
   } catch (const std::exception &e) {
     t_err = e.what();

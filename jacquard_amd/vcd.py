@@ -446,41 +446,28 @@ def write_output_vcd_timed(script, offsets_timestamps, states, arrival_states,
                            timescale=(1, "ps"), scope="gem_top_module",
                            xmask_states=None):
     """Timed VCD (src/sim/vcd_io.rs:831-946): every transition is shifted by
-    its word's arrival (ps, rounded to VCD units), transitions are stably
-    sorted by time."""
+    its word's arrival (ps, rounded half away from zero to VCD units) and
+    transitions are stably sorted by time.  Values and arrivals are gathered
+    here; the host library writes the value-change section."""
     mapping = output_mapping(script)
-    lines = _header_text(timescale, scope, mapping)
     if timescale:
         ps_per_unit = timescale[0] * (_PS_DIVISOR["ps"] / _PS_DIVISOR[timescale[1]])
     else:
         ps_per_unit = 1.0
-    last = [2] * len(mapping)
-    trans = []
     rows = [(int(o), int(t)) for o, t in offsets_timestamps if t != U64_MAX]
-    allv = _output_values(mapping, [o for o, _ in rows], states, xmask_states)
-    for r, (offset, ts) in enumerate(rows):
-        for i, v in enumerate(allv[r].tolist()):
-            if v == last[i]:
-                continue
-            last[i] = v
-            pos = mapping[i][1]
-            arr = 0 if pos is None else int(arrival_states[offset + (pos >> 5)]) & 0xFFFF
-            units = int(_round_half_away(arr / ps_per_unit)) if ps_per_unit > 0 else 0
-            trans.append((ts + units, i, v))
-    trans.sort(key=lambda x: x[0])
-    cur = None
-    for ts, i, v in trans:
-        if ts != cur:
-            lines.append(f"#{ts}")
-            cur = ts
-        lines.append(f"{_SYM[v]}{id_code(i)}")
-    return "\n".join(lines) + "\n"
-
-
-def _round_half_away(x):
-    """f64::round semantics (half away from zero)."""
-    import math
-    return math.floor(x + 0.5) if x >= 0 else -math.floor(-x + 0.5)
+    offs = np.asarray([o for o, _ in rows], np.int64)
+    vals = _output_values(mapping, offs, states, xmask_states)
+    arr = np.zeros(vals.shape, np.uint16)
+    ast = np.asarray(arrival_states, np.uint32)
+    for i, (_, pos, _) in enumerate(mapping):
+        if pos is not None:
+            arr[:, i] = (ast[offs + (pos >> 5)] & np.uint32(0xFFFF)).astype(np.uint16)
+    ts = np.asarray([t for _, t in rows], np.uint64)
+    codes, off = _codes(len(mapping))
+    h = N.host().jqh_vcd_write_timed(vals.ctypes.data, arr.ctypes.data, vals.shape[0],
+                                     vals.shape[1], ts.ctypes.data, float(ps_per_unit),
+                                     codes, off.ctypes.data)
+    return "\n".join(_header_text(timescale, scope, mapping)) + "\n" + _native_text(h)
 
 
 # ---------------------------------------------------------------- buffers

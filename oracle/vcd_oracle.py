@@ -144,3 +144,38 @@ def body_from_changes(counts, records, timestamps):
             out.append(f"{_SYM[rec & 3]}{id_code(rec >> 2)}")
         k += int(counts[r])
     return "".join(line + "\n" for line in out)
+
+
+def _round_half_away(x):
+    """f64::round: halves away from zero."""
+    import math
+    return math.floor(x + 0.5) if x >= 0 else -math.floor(-x + 0.5)
+
+
+def body_timed(rows, arrivals, timestamps, ps_per_unit):
+    """Timed form (src/sim/vcd_io.rs:831-946): a change of column i at row r
+    happens at timestamps[r] + round(arrivals[r][i] / ps_per_unit) (0 when
+    ps_per_unit <= 0); changes are stably sorted by time and '#t' opens
+    each distinct time."""
+    trans = []
+    last = None
+    for r, (row, ts) in enumerate(zip(rows, timestamps)):
+        if last is None:
+            last = [2] * len(row)
+        for i, v in enumerate(row):
+            v = int(v)
+            if v == last[i]:
+                continue
+            last[i] = v
+            a = int(arrivals[r][i])
+            units = int(_round_half_away(a / ps_per_unit)) if ps_per_unit > 0 else 0
+            trans.append((int(ts) + units, i, v))
+    trans.sort(key=lambda x: x[0])
+    out = []
+    cur = None
+    for t, i, v in trans:
+        if t != cur:
+            out.append(f"#{t}")
+            cur = t
+        out.append(f"{_SYM[v]}{id_code(i)}")
+    return "".join(line + "\n" for line in out)

@@ -95,3 +95,31 @@ def test_lane_writer_matches_the_snapshot_writer():
     ts = [10 * (k + 1) for k in range(c)]
     want = vcd.write_output_vcd(s, [((k + 1) * rio, ts[k]) for k in range(c)], st)
     assert vcd.write_lane_vcd(s, obs, lits, ts) == want
+
+
+@pytest.mark.parametrize("ps_per_unit", [1.0, 1000.0, 0.001, 333.0, 0.0])
+def test_timed_body_matches_the_restatement(ps_per_unit):
+    """Arrival-shifted changes, stably sorted by time (ties keep row and
+    column order), with halves rounded away from zero and times past
+    2^64 - 1 near the end of the range."""
+    from jacquard_amd import _native as N
+    rng = np.random.default_rng(int(ps_per_unit * 7) + 1)
+    rows, cols = 150, 120
+    vals = rng.choice(np.array([0, 1, 3], np.uint8), size=(rows, cols), p=[0.45, 0.45, 0.1])
+    keep = rng.random((rows, cols)) < 0.6
+    for r in range(1, rows):
+        vals[r][keep[r]] = vals[r - 1][keep[r]]
+    arr = rng.integers(0, 65536, (rows, cols)).astype(np.uint16)
+    arr[:, ::7] = 500  # exact halves at 1000 ps per unit
+    ts = np.arange(rows, dtype=np.uint64) * np.uint64(40)
+    ts[-20:] = np.uint64(2**64 - 1) - np.arange(20, 0, -1, dtype=np.uint64)
+    codes, off = vcd._codes(cols)
+    h = N.host().jqh_vcd_write_timed(vals.ctypes.data, arr.ctypes.data, rows, cols,
+                                     ts.ctypes.data, ps_per_unit, codes, off.ctypes.data)
+    got = vcd._native_text(h)
+    want = vcd_oracle.body_timed(vals.tolist(), arr.tolist(), ts.tolist(), ps_per_unit)
+    assert got == want
+    times = [int(line[1:]) for line in got.splitlines() if line.startswith("#")]
+    assert times == sorted(times)
+    if ps_per_unit > 0:
+        assert max(times) > 2**64 - 1, "some shift passes the end of the u64 range"
