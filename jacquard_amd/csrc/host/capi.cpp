@@ -274,9 +274,9 @@ void *jqh_flatten(void *aig, void *compiled, uint32_t num_blocks, int xprop,
   // come from are not needed after flattening
   auto pairs = [](std::vector<uint32_t> &dense, std::vector<uint32_t> &key,
                   std::vector<uint32_t> &val) {
-    for (uint32_t k = 0; k < dense.size(); ++k)
+    for (size_t k = 0; k < dense.size(); ++k)
       if (dense[k] != kNone) {
-        key.push_back(k);
+        key.push_back((uint32_t)k);
         val.push_back(dense[k]);
       }
     std::vector<uint32_t>().swap(dense);
