@@ -966,7 +966,7 @@ void merge_partitions(const Aig &g, const Compiled &in, const uint32_t *pairs,
     };
     std::vector<Partition> res(pl.size());
     std::vector<uint8_t> ok(pl.size(), 0);
-    std::string par_err;
+    std::vector<std::string> par_err(pl.size());
 #pragma omp parallel
     {
       int tid = 0;
@@ -991,12 +991,12 @@ void merge_partitions(const Aig &g, const Compiled &in, const uint32_t *pairs,
           std::sort(eps.begin(), eps.end());
           ok[k] = map_group(g, in.opt, ms, infos, avail, eps, *ctxs[tid], res[k]) ? 1 : 0;
         } catch (const std::exception &e) {
-#pragma omp critical(jq_merge_err)
-          if (par_err.empty()) par_err = e.what();
+          par_err[k] = e.what();
         }
       }
     }
-    if (!par_err.empty()) throw std::runtime_error(par_err);
+    for (const std::string &e : par_err)  // the first failure in pair order
+      if (!e.empty()) throw std::runtime_error(e);
     std::vector<uint8_t> drop(ms.parts.size(), 0);
     for (size_t k = 0; k < pl.size(); ++k)
       if (ok[k]) {
@@ -1061,7 +1061,7 @@ void split_partitions(const Aig &g, const Compiled &in, const uint32_t *parts,
     };
     std::vector<Partition> lo(pl.size()), hi(pl.size());
     std::vector<uint8_t> ok(pl.size(), 0);
-    std::string par_err;
+    std::vector<std::string> par_err(pl.size());
 #pragma omp parallel
     {
       int tid = 0;
@@ -1089,12 +1089,12 @@ void split_partitions(const Aig &g, const Compiled &in, const uint32_t *parts,
                       ? 1
                       : 0;
         } catch (const std::exception &e) {
-#pragma omp critical(jq_split_err)
-          if (par_err.empty()) par_err = e.what();
+          par_err[k] = e.what();
         }
       }
     }
-    if (!par_err.empty()) throw std::runtime_error(par_err);
+    for (const std::string &e : par_err)  // the first failure in partition order
+      if (!e.empty()) throw std::runtime_error(e);
     std::vector<uint8_t> drop(ms.parts.size(), 0);
     for (size_t k = 0; k < pl.size(); ++k)
       if (ok[k]) {
