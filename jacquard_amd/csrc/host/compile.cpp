@@ -297,6 +297,11 @@ bool map_partition_ml(const Aig &g, const MapInput &in, Scratch &sc,
   constexpr uint32_t kInf = 0xFFFFFFFFu;
   auto capl = [](int k) -> uint32_t { return k == 0 ? (uint32_t)kWidth : (uint32_t)kWidth >> k; };
   std::vector<std::vector<uint32_t>> occ(kBoomStages + 1);
+  // per level, the length of the run of occupied slots at the front and at
+  // the back: first-fit scans start past it.  Slots are freed only by a
+  // rollback (slots that were free before the attempt, so never inside a
+  // run), by a new stage and by a deferred root; the last two reset these.
+  std::vector<uint32_t> front_run(kBoomStages + 1, 0), back_run(kBoomStages + 1, 0);
   std::vector<uint32_t> depth(n, kInf), now(n, 0);
   std::vector<std::pair<uint8_t, uint16_t>> log;
   // place value x at node (k, q) with its whole subtree
@@ -336,6 +341,8 @@ bool map_partition_ml(const Aig &g, const MapInput &in, Scratch &sc,
     if (!todo) break;
     if (out.size() >= 4096) return false;
     for (int k = 0; k <= kBoomStages; ++k) occ[k].assign(capl(k), kNone);
+    std::fill(front_run.begin(), front_run.end(), 0u);
+    std::fill(back_run.begin(), back_run.end(), 0u);
     std::fill(now.begin(), now.end(), 0u);
     log.clear();
     // cone depth of every gate not computed yet, over present values
@@ -361,6 +368,8 @@ bool map_partition_ml(const Aig &g, const MapInput &in, Scratch &sc,
     std::vector<uint8_t> wrote_now(n, 0);
     auto reset_stage = [&]() {
       for (int k = 0; k <= kBoomStages; ++k) std::fill(occ[k].begin(), occ[k].end(), kNone);
+      std::fill(front_run.begin(), front_run.end(), 0u);
+      std::fill(back_run.begin(), back_run.end(), 0u);
       std::fill(now.begin(), now.end(), 0u);
       log.clear();
       wpos.clear();
@@ -386,7 +395,11 @@ bool map_partition_ml(const Aig &g, const MapInput &in, Scratch &sc,
       uint32_t cap = capl(k);
       size_t mark = log.size();
       uint32_t tries = 0;
-      for (uint32_t j = 0; j < cap && tries < max_tries; ++j) {
+      // occupied slots are skipped without counting as tries, so the scan
+      // may start past the occupied run at its end
+      uint32_t &run = is_out[x] ? front_run[k] : back_run[k];
+      while (run < cap && occ[k][is_out[x] ? run : cap - 1 - run] != kNone) ++run;
+      for (uint32_t j = run; j < cap && tries < max_tries; ++j) {
         uint32_t q = is_out[x] ? j : cap - 1 - j;
         if (occ[k][q] != kNone) continue;
         ++tries;
@@ -497,6 +510,8 @@ bool map_partition_ml(const Aig &g, const MapInput &in, Scratch &sc,
             occ[e.first][e.second] = kNone;
           }
           roots.erase(roots.begin() + (long)r);
+          std::fill(front_run.begin(), front_run.end(), 0u);
+          std::fill(back_run.begin(), back_run.end(), 0u);
           ++free_slots;
         }
         if (need > free_slots) return false;
