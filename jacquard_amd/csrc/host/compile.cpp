@@ -797,7 +797,7 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
     // greedy grouping of endpoints; a group's cone is tracked with stamps
     std::vector<uint32_t> stack;
     std::vector<uint32_t> cur_nodes, cur_inputs;
-    std::vector<uint32_t> lvl_count;
+    std::vector<uint32_t> lvl_count, lvl_extra, lvl_touched;
     uint32_t cur_outs = 0, cur_srams = 0;
     std::vector<uint32_t> add_nodes, add_inputs;
     std::vector<uint32_t> &mark = mark_all;  // group membership stamp
@@ -849,12 +849,18 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
         if (cur_nodes.size() + add_nodes.size() > opt.max_part_nodes) fits = false;
         if (cur_srams + infos[e].nsram > 48) fits = false;
         if (fits) {
-          std::map<uint32_t, uint32_t> extra;
-          for (auto p : add_nodes) ++extra[lvl_m[p]];
-          for (auto &kv : extra) {
-            uint32_t have = kv.first < lvl_count.size() ? lvl_count[kv.first] : 0;
-            if (have + kv.second > opt.max_level_nodes) fits = false;
+          // nodes the endpoint adds per stage-relative level, against the cap
+          for (auto p : add_nodes) {
+            const uint32_t l = lvl_m[p];
+            if (l >= lvl_extra.size()) lvl_extra.resize(l + 1, 0);
+            if (lvl_extra[l]++ == 0) lvl_touched.push_back(l);
           }
+          for (uint32_t l : lvl_touched) {
+            const uint32_t have = l < lvl_count.size() ? lvl_count[l] : 0;
+            if (have + lvl_extra[l] > opt.max_level_nodes) fits = false;
+            lvl_extra[l] = 0;
+          }
+          lvl_touched.clear();
         }
         if (!fits) {
           flush();
