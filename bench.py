@@ -717,7 +717,7 @@ def bench_mult16m(a, torch):
     (trained on >=50K samples of the kernel's own partition timings) and,
     for the makespan comparison, with the heuristic one; blocks per group
     default to the co-resident maximum for the lane groups (no width sweep:
-    each candidate's compile and flattening of this script take about 7 s
+    each candidate's compile and flattening of this script take about 6 s
     on 8 host cores).  The plain
     drop-in runs --steps x --cycles-per-step cycles beside it."""
     from jacquard_amd.netlist import Netlist
@@ -784,7 +784,7 @@ def forms_16m(a, torch):
     x 20,000 cycles through 2 groups x 128 blocks at level split 5 with the
     learned block
     assignment (A/B against the heuristic; no merge and no tiling sweep,
-    whose compile and flattening take about 7 s per candidate) in one
+    whose compile and flattening take about 6 s per candidate) in one
     graph replay, and the plain drop-in (compiled at level split 4) over
     2 x 1,000 cycles."""
     import ctypes

@@ -5,6 +5,7 @@
 #include "compile.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstring>
 #include <functional>
@@ -1243,8 +1244,65 @@ uint32_t get_or_place(FlatPart &fp, uint32_t lit, uint32_t clken_lit) {
   return r;
 }
 
+// Open-addressing map for flatten's per-partition tables: linear probing
+// over a power-of-two array kept at most half full, slots by Fibonacci
+// hashing (the key times 2^64 / golden ratio, high bits).  The all-ones key
+// is never stored: it marks empty slots.
+template <class K, class V>
+class FlatMap {
+ public:
+  void clear() {
+    if (used_) std::fill(keys_.begin(), keys_.end(), kEmpty);
+    used_ = 0;
+  }
+  const V *find(K k) const {
+    if (keys_.empty()) return nullptr;
+    for (size_t i = slot(k);; i = (i + 1) & (keys_.size() - 1)) {
+      if (keys_[i] == k) return &vals_[i];
+      if (keys_[i] == kEmpty) return nullptr;
+    }
+  }
+  void set(K k, V v) {
+    if (2 * (used_ + 1) > keys_.size()) grow();
+    size_t i = slot(k);
+    while (keys_[i] != kEmpty && keys_[i] != k) i = (i + 1) & (keys_.size() - 1);
+    if (keys_[i] == kEmpty) {
+      keys_[i] = k;
+      ++used_;
+    }
+    vals_[i] = v;
+  }
+
+ private:
+  static constexpr K kEmpty = (K)~(K)0;
+  std::vector<K> keys_;
+  std::vector<V> vals_;
+  size_t used_ = 0;
+  uint32_t shift_ = 64;
+  size_t slot(K k) const {
+    return (size_t)(((uint64_t)k * 0x9E3779B97F4A7C15ull) >> shift_);
+  }
+  void grow() {
+    std::vector<K> ok;
+    std::vector<V> ov;
+    ok.swap(keys_);
+    ov.swap(vals_);
+    const size_t cap = ok.empty() ? 1024 : 2 * ok.size();
+    shift_ = 64 - (uint32_t)__builtin_ctzll(cap);
+    keys_.assign(cap, kEmpty);
+    vals_.assign(cap, V());
+    used_ = 0;
+    for (size_t j = 0; j < ok.size(); ++j)
+      if (ok[j] != kEmpty) set(ok[j], ov[j]);
+  }
+};
+
 // Pack (word, mask) global reads into 256 threads x rounds, <= 32 bits per
 // thread (contract of map_global_read_to_rounds, src/flatten.rs:343-400).
+// Each entry goes to the thread with the fewest entries, then the most bits
+// that still leave room for it, then the lowest index; threads are kept in
+// 256-bit sets by (entries, bits), so the choice is a walk over those sets
+// in that order instead of a scan of all threads.
 void pack_global_reads(std::vector<std::pair<uint32_t, uint32_t>> entries,
                        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> &th) {
   th.assign(kThreads, {});
@@ -1253,16 +1311,28 @@ void pack_global_reads(std::vector<std::pair<uint32_t, uint32_t>> entries,
     return __builtin_popcount(x.second) > __builtin_popcount(y.second);
   });
   std::vector<std::pair<uint32_t, uint32_t>> work(entries.rbegin(), entries.rend());
+  // a thread holds at most 32 entries (each has a bit) and 32 bits
+  constexpr uint32_t kLim = 33;
+  static_assert(kThreads == 256, "four 64-bit words per thread set");
+  std::vector<std::array<uint64_t, 4>> set(kLim * kLim, std::array<uint64_t, 4>{0, 0, 0, 0});
+  std::array<uint32_t, kLim> with_size{};
+  set[0] = {~0ull, ~0ull, ~0ull, ~0ull};
+  with_size[0] = kThreads;
   while (!work.empty()) {
     auto e = work.back();
     work.pop_back();
     uint32_t pc = (uint32_t)__builtin_popcount(e.second);
     int best = -1;
-    for (uint32_t t = 0; t < kThreads; ++t) {
-      if (bits[t] + pc > 32) continue;
-      if (best < 0 || th[t].size() < th[best].size() ||
-          (th[t].size() == th[best].size() && bits[t] > bits[best]))
-        best = (int)t;
+    for (uint32_t sz = 0; sz < kLim && best < 0; ++sz) {
+      if (!with_size[sz]) continue;
+      for (int b = 32 - (int)pc; b >= 0 && best < 0; --b) {
+        const auto &w = set[sz * kLim + (uint32_t)b];
+        for (int q = 0; q < 4; ++q)
+          if (w[q]) {
+            best = q * 64 + __builtin_ctzll(w[q]);
+            break;
+          }
+      }
     }
     if (best < 0) {
       // split the mask in two halves
@@ -1277,8 +1347,14 @@ void pack_global_reads(std::vector<std::pair<uint32_t, uint32_t>> entries,
       work.push_back({e.first, lo});
       continue;
     }
+    const uint32_t sz = (uint32_t)th[best].size(), bq = (uint32_t)best >> 6;
+    const uint64_t bm = 1ull << (best & 63);
+    set[sz * kLim + bits[best]][bq] &= ~bm;
+    --with_size[sz];
     th[best].push_back(e);
     bits[best] += pc;
+    set[(sz + 1) * kLim + bits[best]][bq] |= bm;
+    ++with_size[sz + 1];
   }
 }
 
@@ -1622,14 +1698,25 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
       while (sc.size() < 256) sc.push_back(0xFFFFFFFFu);
       if (sc.size() != 256) throw std::runtime_error("flatten: too many writeout hooks");
       // global reads
-      std::map<uint32_t, uint32_t> taken;  // word|flag -> mask
+      // (word | flag, mask) per word read, in word order
+      std::vector<std::pair<uint32_t, uint32_t>> ent;
+      ent.reserve(kWidth);
       for (uint32_t pin : pt.stages[0].hier[0]) {
         if (pin == kNone) continue;
         bool staged;
         const uint32_t pos = read_pos(pin, staged);
-        taken[(pos >> 5) | (staged ? 0x80000000u : 0u)] |= 1u << (pos & 31);
+        ent.push_back({(pos >> 5) | (staged ? 0x80000000u : 0u), 1u << (pos & 31)});
       }
-      std::vector<std::pair<uint32_t, uint32_t>> ent(taken.begin(), taken.end());
+      std::sort(ent.begin(), ent.end());
+      {
+        size_t o = 0;
+        for (size_t k = 0; k < ent.size(); ++k)
+          if (o && ent[o - 1].first == ent[k].first)
+            ent[o - 1].second |= ent[k].second;
+          else
+            ent[o++] = ent[k];
+        ent.resize(o);
+      }
       std::vector<std::vector<std::pair<uint32_t, uint32_t>>> th;
       pack_global_reads(ent, th);
       uint32_t rounds = 0;
@@ -1637,7 +1724,7 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
       sc[6] = rounds;
       size_t gstart = sc.size();
       sc.resize(gstart + 2 * (size_t)rounds * kThreads, 0);
-      std::unordered_map<uint64_t, uint16_t> local;  // (flag,pos) -> local bit
+      FlatMap<uint64_t, uint16_t> local;  // (flag,pos) -> local bit
       for (uint32_t t = 0; t < kThreads; ++t) {
         for (size_t r = 0; r < th[t].size(); ++r) {
           sc[gstart + kThreads * 2 * r + 2 * t] = th[t][r].first;
@@ -1650,24 +1737,31 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
           for (int k = 31; k >= 0; --k)
             if (mask >> k & 1) {
               uint64_t key = flag << 40 | (uint64_t)((idx & 0x7FFFFFFFu) * 32ull + k);
-              local[key] = (uint16_t)(t * 32 + bit_id);
+              local.set(key, (uint16_t)(t * 32 + bit_id));
               ++bit_id;
             }
         }
       }
-      std::unordered_map<uint32_t, uint16_t> last_pos;
+      FlatMap<uint32_t, uint16_t> last_pos;  // pin -> position in the layer
+      auto pos_of = [&last_pos](uint32_t pin) -> uint16_t {
+        const uint16_t *v = last_pos.find(pin);
+        if (!v) throw std::runtime_error("flatten: a tree operand has no position");
+        return *v;
+      };
       for (uint32_t pin : pt.stages[0].hier[0]) {
-        if (pin == kNone || last_pos.count(pin)) continue;
+        if (pin == kNone || last_pos.find(pin)) continue;
         bool staged;
         const uint32_t pos = read_pos(pin, staged);
-        last_pos[pin] = local.at((uint64_t)staged << 40 | pos);
+        const uint16_t *lb = local.find((uint64_t)staged << 40 | pos);
+        if (!lb) throw std::runtime_error("flatten: a read bit has no local position");
+        last_pos.set(pin, *lb);
       }
       for (size_t j = 0; j < pt.stages.size(); ++j) {
         const BoomStage &bs = pt.stages[j];
         std::vector<uint16_t> perm(kWidth, 0);
         for (uint32_t i2 = 0; i2 < kWidth; ++i2) {
           uint32_t pin = bs.hier[0][i2];
-          if (pin != kNone) perm[i2] = last_pos.at(pin);
+          if (pin != kNone) perm[i2] = pos_of(pin);
         }
         std::vector<uint32_t> xa(kThreads, 0), xb(kThreads, 0), ob(kThreads, 0);
         std::vector<std::vector<uint32_t>> tpins(kThreads);
@@ -1715,7 +1809,7 @@ void flatten(const Aig &g, const Compiled &c, const CompileOptions &opt,
           uint32_t len = kWidth >> h;
           for (uint32_t q = 0; q < len; ++q) {
             uint32_t o = bs.hier[h][q];
-            if (o != kNone) last_pos[o] = (uint16_t)(len + q);
+            if (o != kNone) last_pos.set(o, (uint16_t)(len + q));
           }
         }
       }
