@@ -222,7 +222,7 @@ void *jqh_vcd_write_values(const uint8_t *vals, uint64_t n_rows, uint32_t n_cols
   auto *out = new VcdText();
   try {
     std::vector<uint8_t> last(n_cols, 2);
-    out->s.reserve(n_rows * 8 + n_rows * (size_t)n_cols);
+    out->s.reserve(n_rows * 12);  // the time lines; changes grow it
     for (uint64_t r = 0; r < n_rows; ++r) {
       out->s.push_back('#');
       put_u64(out->s, timestamps[r]);

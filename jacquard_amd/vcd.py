@@ -288,9 +288,6 @@ def _header_text(timescale, scope, mapping):
     return lines
 
 
-_SYM = {0: "0", 1: "1", 3: "x"}
-
-
 def _codes(n):
     """Identifier codes of columns 0..n-1, concatenated, and their offsets."""
     codes = [id_code(i) for i in range(n)]
