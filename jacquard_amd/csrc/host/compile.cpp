@@ -305,8 +305,9 @@ bool map_partition_ml(const Aig &g, const MapInput &in, Scratch &sc,
   std::vector<uint32_t> front_run(kBoomStages + 1, 0), back_run(kBoomStages + 1, 0);
   std::vector<uint32_t> depth(n, kInf), now(n, 0);
   std::vector<std::pair<uint8_t, uint16_t>> log;
-  // place value x at node (k, q) with its whole subtree
-  std::function<bool(uint32_t, int, uint32_t)> put = [&](uint32_t x, int k, uint32_t q) -> bool {
+  // place value x at node (k, q) with its whole subtree (recursing through
+  // a generic lambda rather than a std::function, so the calls inline)
+  auto put_at = [&](auto &self, uint32_t x, int k, uint32_t q) -> bool {
     if (occ[k][q] != kNone) return false;
     if (k == 0) {
       if (!present[x]) return false;
@@ -317,9 +318,10 @@ bool map_partition_ml(const Aig &g, const MapInput &in, Scratch &sc,
     if (!present[x] && (depth[x] == kInf || depth[x] > (uint32_t)k)) return false;
     occ[k][q] = x;
     log.push_back({(uint8_t)k, (uint16_t)q});
-    if (present[x]) return put(x, k - 1, q);
-    return put(opa[x], k - 1, q) && put(opb[x], k - 1, q + capl(k));
+    if (present[x]) return self(self, x, k - 1, q);
+    return self(self, opa[x], k - 1, q) && self(self, opb[x], k - 1, q + capl(k));
   };
+  auto put = [&put_at](uint32_t x, int k, uint32_t q) { return put_at(put_at, x, k, q); };
   // undo a placement attempt that failed (its nodes were never counted)
   auto rollback = [&](size_t mark) {
     while (log.size() > mark) {
