@@ -122,6 +122,23 @@ class Compiled:
         return cache[num_blocks]
 
 
+class _ScriptHandle:
+    """Owns a flattened script's native handle.  blocks_data is a view of the
+    handle's buffer and holds this owner too, so the view stays valid after
+    the script object itself is gone."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        h, self.h = self.h, None
+        if h:
+            try:
+                N.host().jqh_script_free(h)
+            except Exception:  # interpreter shutdown: the library is gone
+                pass
+
+
 class FlattenedScriptV1:
     def __init__(self, compiled: Compiled, num_blocks, xprop=False, costs=None):
         self.compiled = compiled
@@ -137,6 +154,7 @@ class FlattenedScriptV1:
                           int(bool(xprop)), cptr)
         if not h:
             raise RuntimeError("flatten failed: " + N.host_error())
+        self._owner = _ScriptHandle(h)
         self._h = h
         info = (ctypes.c_uint64 * 15)()
         L.jqh_script_info(h, info)
@@ -155,8 +173,9 @@ class FlattenedScriptV1:
             ctypes.cast(L.jqh_script_blocks_start(h), ctypes.POINTER(ctypes.c_uint64)),
             (nbs,)).copy()
         self._bd_ptr = L.jqh_script_blocks_data(h)
-        self.blocks_data = np.ctypeslib.as_array(
-            ctypes.cast(self._bd_ptr, ctypes.POINTER(ctypes.c_uint32)), (nbd,))
+        buf = (ctypes.c_uint32 * nbd).from_address(self._bd_ptr)
+        buf._owner = self._owner  # the view keeps the native buffer alive
+        self.blocks_data = np.ctypeslib.as_array(buf)
         nl = int(info[8])
         self.input_layout = np.ctypeslib.as_array(
             ctypes.cast(L.jqh_script_input_layout(h), ctypes.POINTER(ctypes.c_uint32)),
@@ -191,15 +210,6 @@ class FlattenedScriptV1:
         self.sram_offsets = arr(L.jqh_script_sram_offsets(h), nsr)
         nd = self.netlist.counts()["num_dffs"]
         self.dff_data_pos = arr(L.jqh_script_dff_data_pos(h), nd)
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            try:
-                N.host().jqh_script_free(h)
-            except Exception:  # interpreter shutdown: the library is gone
-                pass
-            self._h = None
 
     # ------------------------------------------------------------------
     @property

@@ -240,3 +240,23 @@ def test_an_endpoint_wider_than_a_partition_is_an_error_and_the_compiler_recover
     st = stimulus.build_states(s, cycles, seed=2, p=0.999)
     oracle.run(s, st, cycles)
     direct_check(s, st, cycles, 2, 0.999)
+
+
+def test_blocks_data_outlives_its_script_object():
+    """blocks_data is a view of the host library's buffer: it keeps that
+    buffer alive after the script object that made it is gone (it used to
+    dangle, so reading it then touched freed memory)."""
+    import gc
+    from jacquard_amd.netlist import Netlist
+    from jacquard_amd.script import Compiled
+    nl = Netlist.random(seed=5, gates=20000, depth=24, dffs=512, inputs=32, outputs=16,
+                        geometric_p=0.3)
+    comp = Compiled(nl, level_split=4)
+    views = [comp.flatten(16).blocks_data for _ in range(3)]
+    gc.collect()
+    # same-size blocks, so the allocator hands out any freed buffer again
+    churn = [np.full(views[0].size, 0xA5A5A5A5, np.uint32) for _ in range(16)]
+    ref = comp.flatten(16)
+    for v in views:
+        assert np.array_equal(v, ref.blocks_data)
+    del churn

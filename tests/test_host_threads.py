@@ -1,14 +1,16 @@
 """The host compile pipeline's parallel paths give the same script as one
 thread.
 
-Grouping runs on one thread while the groups it closes are mapped as tasks;
+Grouping runs on one thread while the groups it closes are mapped as tasks,
+and partition merge and split map their candidates in parallel;
 flattening places every partition of a major stage in parallel, records its
 map writes and applies them in partition order, then builds and copies the
 sections in parallel.  With OMP_NUM_THREADS=1 all of that runs in program
 order, so comparing a run with one thread against runs with several checks
 that no result depends on which thread finishes first: every byte of the
 script, the three maps, the SRAM offsets, the DFF data positions, the
-partition records, X capability and the injected delays."""
+partition records, X capability and the injected delays, and the merged and
+split designs' scripts."""
 import hashlib
 import json
 import os
@@ -46,11 +48,15 @@ for name, kw, split, nb, xp in [
     c = Compiled(nl, level_split=split)
     s = c.flatten(nb, xprop=xp)
     s.inject_timing()
+    pps = c.parts_per_stage()
+    mg, merged = c.merged([(m, 0, 1) for m in range(len(pps)) if pps[m] >= 2])
+    sp, split = c.split([(m, 0) for m in range(len(pps))])
     out[name] = [h(s.blocks_data), h(s.blocks_start), h(s.input_layout),
                  dh(s.input_map), dh(s.output_map), dh(s.staged_io_map),
                  h(s.parts), h(s.partition_x_capable), h(s.sram_offsets),
-                 h(s.dff_data_pos), s.num_parts, s.reg_io_state_size,
-                 c.parts_per_stage()]
+                 h(s.dff_data_pos), s.num_parts, s.reg_io_state_size, pps,
+                 h(mg.flatten(nb).blocks_data), h(np.asarray(merged, np.uint8)),
+                 h(sp.flatten(nb).blocks_data), h(np.asarray(split, np.uint8))]
 print(json.dumps(out))
 """
 
