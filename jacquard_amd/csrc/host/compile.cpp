@@ -737,9 +737,7 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
   }
   for (size_t e = 0; e < ne; ++e) out.stages[ep_stage[e]].orig_endpoints.push_back((uint32_t)e);
 
-  std::vector<uint32_t> mark_all(g.num_pins + 1, 0);
-  std::vector<uint32_t> seen(g.num_pins + 1, 0);
-  uint32_t gstamp = 1, sstamp = 1;
+  // stage-relative level of every AND node; each stage writes only its own
   std::vector<uint32_t> lvl_m(g.num_pins + 1, 0);
   size_t total_parts = 0, total_bstages = 0, total_nodes = 0;
 
@@ -750,16 +748,26 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
 #endif
   std::vector<std::unique_ptr<MapCtx>> ctxs(max_threads);
 
-  // Grouping is greedy and runs stage by stage on one thread; every group
-  // it closes is mapped as a task by the other threads meanwhile, so the
-  // grouping of later stages overlaps the mapping of earlier ones.  Errors
+  // Each major stage's greedy grouping is a task; once it has closed its
+  // groups it adds one mapping task per group.  A stage's grouping reads
+  // and writes only its own nodes' levels, and tracks membership with the
+  // stamp arrays of the mapping context of the thread running it (its
+  // stamps come from the same counters, so marks never collide).  Errors
   // are kept per stage and group and the first in stage order is rethrown.
   std::vector<std::vector<EndpointInfo>> infos_all(nmajor);
   std::vector<std::vector<std::vector<uint32_t>>> groups_all(nmajor);
   std::vector<std::vector<std::vector<Partition>>> built_all(nmajor);
   std::vector<std::vector<std::string>> err_all(nmajor);
-  std::string group_err;
-  uint32_t group_err_stage = nmajor;
+  std::vector<std::string> group_err(nmajor);
+  std::vector<double> group_s(nmajor, 0.0);
+  auto ctx_here = [&]() -> MapCtx & {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    if (!ctxs[tid]) ctxs[tid] = std::make_unique<MapCtx>(g.num_pins + 1);
+    return *ctxs[tid];
+  };
   // map endpoints `eps` of stage m into res, in halves when they do not fit
   std::function<void(uint32_t, MapCtx &, const std::vector<uint32_t> &,
                      std::vector<Partition> &)>
@@ -786,8 +794,14 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
 
 #pragma omp parallel
 #pragma omp single
-  for (uint32_t m = 0; m < nmajor; ++m) try {
+  for (uint32_t m = 0; m < nmajor; ++m) {
+#pragma omp task firstprivate(m)
+  try {
     auto t0 = clk::now();
+    MapCtx &cx = ctx_here();
+    std::vector<uint32_t> &mark = cx.mark, &seen = cx.seen;  // group / collect stamps
+    uint32_t &gstamp = cx.gstamp, &sstamp = cx.sstamp;
+    ++gstamp;  // the first group's stamp: fresh
     const MajorStage &ms = out.stages[m];
     auto avail_pin = [&](uint32_t p) { return stage_of(p) < (int)m; };
     // stage-relative levels (cone depth above this stage's inputs)
@@ -803,7 +817,6 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
     std::vector<uint32_t> lvl_count, lvl_extra, lvl_touched;
     uint32_t cur_outs = 0, cur_srams = 0;
     std::vector<uint32_t> add_nodes, add_inputs;
-    std::vector<uint32_t> &mark = mark_all;  // group membership stamp
     auto collect = [&](size_t e) {
       add_nodes.clear();
       add_inputs.clear();
@@ -885,37 +898,33 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
       cur_srams += infos[e].nsram;
     }
     flush();
-    t_group += secs(t0);
+    group_s[m] = secs(t0);
+    // the grouping is done with the context's stamp arrays: from here on
+    // the thread may run mapping tasks on the same context
     built_all[m].resize(group_eps.size());
     err_all[m].resize(group_eps.size());
     for (size_t gi = 0; gi < group_eps.size(); ++gi) {
 #pragma omp task firstprivate(m, gi)
       {
-        int tid = 0;
-#ifdef _OPENMP
-        tid = omp_get_thread_num();
-#endif
         try {
-          if (!ctxs[tid]) ctxs[tid] = std::make_unique<MapCtx>(g.num_pins + 1);
-          build(m, *ctxs[tid], groups_all[m][gi], built_all[m][gi]);
+          build(m, ctx_here(), groups_all[m][gi], built_all[m][gi]);
         } catch (const std::exception &e) {
           err_all[m][gi] = e.what();
         }
       }
     }
   } catch (const std::exception &e) {
-    // grouping failed: no later stage is grouped (its tasks still drain)
-    group_err = e.what();
-    group_err_stage = m;
-    break;
+    group_err[m] = e.what();
+  }
   }
   // (the parallel region's closing barrier has waited for every task)
   for (uint32_t m = 0; m < nmajor; ++m) {
-    if (m == group_err_stage) throw std::runtime_error(group_err);
+    if (!group_err[m].empty()) throw std::runtime_error(group_err[m]);
     for (const std::string &e : err_all[m])
       if (!e.empty()) throw std::runtime_error(e);
   }
-  t_map = secs(t_all) - t_group;
+  for (double t : group_s) t_group += t;
+  t_map = secs(t_all);
   for (uint32_t m = 0; m < nmajor; ++m) {
     MajorStage &ms = out.stages[m];
     for (auto &v : built_all[m])
@@ -934,8 +943,8 @@ void compile_partitions(const Aig &g, const CompileOptions &opt, Compiled &out,
        << ms.orig_endpoints.size() << " endpoints, " << ms.parts.size()
        << " partitions\n";
   }
-  lg << "compile time: grouping " << t_group << " s, mapping " << t_map
-     << " s beyond the grouping it overlaps\n";
+  lg << "compile time: grouping " << t_group << " s summed over major stages, grouping and "
+     << "mapping " << t_map << " s in all\n";
   lg << "total: " << total_parts << " partitions, " << total_bstages
      << " boomerang stages, " << total_nodes << " mapped AND evaluations ("
      << g.num_and() << " unique)\n";

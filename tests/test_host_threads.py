@@ -1,8 +1,8 @@
 """The host compile pipeline's parallel paths give the same script as one
 thread.
 
-Grouping runs on one thread while the groups it closes are mapped as tasks,
-and partition merge and split map their candidates in parallel;
+Every major stage's grouping is a task and so is the mapping of every group
+it closes, and partition merge and split map their candidates in parallel;
 flattening places every partition of a major stage in parallel, records its
 map writes and applies them in partition order, then builds and copies the
 sections in parallel.  With OMP_NUM_THREADS=1 all of that runs in program
