@@ -171,10 +171,111 @@ class ReportBuilder:
         }
 
 
+def _worst_of_kind(rec, slack, n_top):
+    """ReportBuilder._push_worst over the records `rec` (indices, in
+    canonical order) of one kind: the first n_top fill the list; afterwards
+    a record replaces the last of the list's largest slacks when its slack
+    is smaller.  That largest slack never rises, so once the list is full
+    only records below its value at that point can change it: the others
+    are skipped without changing the result."""
+    items = [int(i) for i in rec[:n_top]]
+    if len(rec) <= n_top:
+        return items
+    mx = max(int(slack[i]) for i in items)
+    rest = rec[n_top:]
+    for i in rest[slack[rest] < mx].tolist():
+        s = int(slack[i])
+        if s >= mx:
+            continue
+        j = max(k for k, it in enumerate(items) if int(slack[it]) == mx)
+        items[j] = i
+        mx = max(int(slack[it]) for it in items)
+    return items
+
+
 def build_report(events, cycles_run, clock_period_ps, design=None,
                  vector_source=None, timing_source=None, describe=default_site,
                  max_violations=DEFAULT_MAX_VIOLATIONS, events_dropped=0,
                  version="0.1.0"):
+    """The report ReportBuilder gives after observing every record in
+    canonical order, computed over the whole event array at once: per-word
+    aggregates by grouping, the bounded record list from its first entries,
+    and the worst-slack lists by the same replacement rule.  describe is
+    called once per word."""
+    from .sim import canonical_events
+    md = dict(design=design, vector_source=vector_source, timing_source=timing_source,
+              clock_period_ps=int(clock_period_ps), cycles_run=int(cycles_run),
+              jacquard_version=version)
+    ev = canonical_events(events)
+    if len(ev):
+        ev = ev[(ev["event_type"] == EVENT_SETUP) | (ev["event_type"] == EVENT_HOLD)]
+    n = len(ev)
+    cyc = ev["cycle"].astype(np.int64)
+    word = ev["data"][:, 0].astype(np.int64)
+    slack = np.ascontiguousarray(ev["data"][:, 1]).view(np.int32).astype(np.int64)
+    arr = ev["data"][:, 2].astype(np.int64)
+    lim = ev["data"][:, 3].astype(np.int64)
+    setup = ev["event_type"] == EVENT_SETUP
+    sites = {}
+
+    def site(w):
+        if w not in sites:
+            sites[w] = describe(w)
+        return sites[w]
+
+    def rec_dict(i):
+        w = int(word[i])
+        return {"cycle": int(cyc[i]), "kind": "setup" if setup[i] else "hold",
+                "word_id": w, "site": site(w), "arrival_ps": int(arr[i]),
+                "constraint_ps": int(lim[i]), "slack_ps": int(slack[i])}
+
+    keep = n if max_violations is None else min(n, max_violations)
+    violations = [rec_dict(i) for i in range(keep)]
+    per_word = []
+    if n:
+        uw, inv = np.unique(word, return_inverse=True)
+        u = uw.size
+        n_setup = np.bincount(inv[setup], minlength=u)
+        n_hold = np.bincount(inv[~setup], minlength=u)
+        big = np.iinfo(np.int64).max
+        ws = np.full(u, big, np.int64)
+        np.minimum.at(ws, inv[setup], slack[setup])
+        wh = np.full(u, big, np.int64)
+        np.minimum.at(wh, inv[~setup], slack[~setup])
+        wa = np.full(u, -1, np.int64)
+        np.maximum.at(wa, inv, arr)
+        for k in np.lexsort((uw, -(n_setup + n_hold))).tolist():
+            w = int(uw[k])
+            per_word.append({
+                "word_id": w, "site": site(w),
+                "setup_violations": int(n_setup[k]), "hold_violations": int(n_hold[k]),
+                "worst_setup_slack_ps": int(ws[k]) if n_setup[k] else None,
+                "worst_hold_slack_ps": int(wh[k]) if n_hold[k] else None,
+                "worst_arrival_ps": int(wa[k])})
+    n_top = max(1, WORST_SLACK_TOP_N)
+    worst = {}
+    for kind, mask in (("setup", setup), ("hold", ~setup)):
+        items = _worst_of_kind(np.flatnonzero(mask), slack, n_top)
+        items.sort(key=lambda i: (int(slack[i]), int(cyc[i])))
+        worst[kind] = [rec_dict(i) for i in items]
+    return {
+        "schema_version": SCHEMA_VERSION,
+        "metadata": md,
+        "stats": {"setup_violations": int(setup.sum()), "hold_violations": int(n - setup.sum()),
+                  "events_dropped": int(events_dropped),
+                  "violations_truncated": n - keep},
+        "violations": violations,
+        "per_word": per_word,
+        "worst_slack": worst,
+    }
+
+
+def build_report_sequential(events, cycles_run, clock_period_ps, design=None,
+                            vector_source=None, timing_source=None, describe=default_site,
+                            max_violations=DEFAULT_MAX_VIOLATIONS, events_dropped=0,
+                            version="0.1.0"):
+    """build_report through ReportBuilder, one record at a time (the
+    reference's incremental builder, src/timing_report.rs)."""
     rb = ReportBuilder(dict(design=design, vector_source=vector_source,
                             timing_source=timing_source,
                             clock_period_ps=int(clock_period_ps), cycles_run=0,

@@ -5,9 +5,10 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from jacquard_amd import _native as N
-from jacquard_amd import events, vcd
+from jacquard_amd import events, sim, vcd
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -124,3 +125,25 @@ def test_report_truncation():
     assert len(rep["violations"]) == 3
     assert rep["stats"]["violations_truncated"] == 2
     assert rep["stats"]["hold_violations"] == 5
+
+
+@pytest.mark.parametrize("n,seed,cap", [(0, 1, None), (1, 2, 0), (37, 3, 7), (5000, 4, None),
+                                        (20000, 5, 1000), (20000, 6, events.DEFAULT_MAX_VIOLATIONS)])
+def test_batch_report_equals_the_incremental_builder(n, seed, cap):
+    """build_report (grouping over the whole array) against ReportBuilder
+    fed one record at a time: the same JSON bytes, on random events with
+    many equal slacks (the worst-slack lists' replacement rule decides
+    between them), other event types mixed in, and record caps."""
+    rng = np.random.default_rng(seed)
+    ev = np.zeros(n, sim.EVENT_DTYPE)
+    ev["event_type"] = rng.choice([1, 4, 4, 5, 5, 7], n)
+    ev["cycle"] = rng.integers(0, 300, n)
+    ev["message_id"] = rng.integers(0, 60, n)
+    ev["data"][:, 0] = ev["message_id"]
+    ev["data"][:, 1] = rng.integers(-40, 0, n).astype(np.int32).view(np.uint32)
+    ev["data"][:, 2] = rng.integers(0, 5000, n)
+    ev["data"][:, 3] = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    kw = dict(describe=lambda w: f"w{w % 7}/{w}", design="d", events_dropped=3)
+    got = events.build_report(ev, 300, 5000, max_violations=cap, **kw)
+    want = events.build_report_sequential(ev, 300, 5000, max_violations=cap, **kw)
+    assert events.report_json(got) == events.report_json(want)
